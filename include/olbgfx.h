@@ -1,0 +1,168 @@
+/*
+ * olbgfx.h -- C ABI of the gfx950 (MI355X) lattice Boltzmann platform.
+ *
+ * The library owns all device memory; callers own host buffers.  Every call
+ * is stream-ordered on the stream given (NULL = the default stream) unless it
+ * says it synchronises.  A handle must not be used from two threads at once.
+ * No C++ types and no HIP types cross this boundary: streams are passed as
+ * `void*` (a hipStream_t).
+ *
+ * Each entry point names the OpenLB 1.8r1 interface it stands in for
+ * (paths relative to the OpenLB source root).
+ */
+#ifndef OLBGFX_H
+#define OLBGFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct olbgfx_lattice olbgfx_lattice;
+typedef struct olbgfx_ensemble olbgfx_ensemble;
+typedef int32_t olbgfx_status;
+
+#define OLBGFX_OK              0
+#define OLBGFX_ERR_INVALID     1  /* bad argument (std::invalid_argument)   */
+#define OLBGFX_ERR_UNSUPPORTED 2  /* dynamics not provided (std::domain_error) */
+#define OLBGFX_ERR_HIP         3  /* HIP runtime error (device::check)       */
+#define OLBGFX_ERR_NOMEM       4  /* device allocation failed               */
+
+/* population storage precision; arithmetic is FP32 for F32/F16/BF16 and
+ * FP64 for F64.  F16/BF16 hold the shifted populations f_i - t_i. */
+#define OLBGFX_STORE_F32  0
+#define OLBGFX_STORE_F16  1
+#define OLBGFX_STORE_BF16 2
+#define OLBGFX_STORE_F64  3
+
+/* host value types for field and population transfers */
+#define OLBGFX_VALUE_F32 0
+#define OLBGFX_VALUE_F64 1
+
+/* dynamics kinds (the closed set this platform provides) */
+#define OLBGFX_NO_DYNAMICS          0  /* dynamics.h:52-63 NoDynamics            */
+#define OLBGFX_BGK                  1  /* dynamics.h:90-99 BGKdynamics           */
+#define OLBGFX_TRT                  2  /* dynamics.h:531-537 TRTdynamics         */
+#define OLBGFX_FORCED_BGK           3  /* dynamics.h:223-230 ForcedBGKdynamics   */
+#define OLBGFX_FORCED_TRT           4  /* dynamics.h:543-549 ForcedTRTdynamics   */
+#define OLBGFX_BOUNCE_BACK          5  /* dynamics.h:558-568 BounceBack          */
+#define OLBGFX_BOUNCE_BACK_VELOCITY 6  /* dynamics.h:635-640 BounceBackVelocity  */
+
+/* per-cell fields */
+#define OLBGFX_FIELD_VELOCITY 0  /* descriptors::VELOCITY, 3 values per cell */
+#define OLBGFX_FIELD_RHO      1  /* momenta::FixedDensity::RHO, 1 value      */
+#define OLBGFX_FIELD_FORCE    2  /* descriptors::FORCE, 3 values per cell    */
+
+#define OLBGFX_MAX_DYNAMICS 16
+
+typedef struct {
+  int32_t nx, ny, nz;    /* core extent (BlockStructureD::_core)            */
+  int32_t padding;       /* overlap width, >= 1 (BlockStructureD::_padding) */
+  int32_t periodic[3];   /* CuboidDecomposition::setPeriodicity             */
+  int32_t storage;       /* OLBGFX_STORE_*                                   */
+} olbgfx_lattice_desc;
+
+typedef struct {
+  int32_t kind;              /* OLBGFX_* dynamics kind                       */
+  double omega;              /* descriptors::OMEGA                            */
+  double magic;              /* collision::TRT::MAGIC                         */
+  double force[3];           /* uniform FORCE unless a FORCE field is set     */
+  double wall_velocity[3];   /* uniform VELOCITY unless a VELOCITY field is set */
+  double wall_rho;           /* uniform RHO unless a RHO field is set (1)     */
+} olbgfx_dynamics_desc;
+
+/* ---- lattice lifetime --------------------------------------------------
+ * Replaces ConcreteBlockLattice<T,D,GPU_*> construction, which allocates one
+ * cyclic column per population (core/blockLattice.hh:392-415,
+ * core/platform/gpu/hip/column.hh:217-271).  All populations start at 0
+ * (rest equilibrium with rho = 1) and every cell has NoDynamics. */
+olbgfx_status olbgfx_lattice_create(const olbgfx_lattice_desc* desc, olbgfx_lattice** out);
+olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* lattice);
+
+/* number of cells of the padded block (BlockStructureD::getNcells,
+ * core/blockStructure.h:137-148) */
+olbgfx_status olbgfx_lattice_ncells(const olbgfx_lattice* lattice, uint64_t* ncells);
+
+/* neighbour distances d_i = c_i . projection
+ * (BlockStructureD::getNeighborDistance, core/blockStructure.h:180-182) */
+olbgfx_status olbgfx_neighbor_distances(const olbgfx_lattice* lattice, int64_t out[19]);
+
+/* current ring offsets s_i in [0, ncells): population i of cell n is stored
+ * at slot (n + s_i) mod ncells (CyclicColumn::operator[],
+ * core/platform/cpu/sisd/column.h:162-170).  Synchronises. */
+olbgfx_status olbgfx_ring_shifts(olbgfx_lattice* lattice, int64_t out[19]);
+
+/* ---- dynamics ------------------------------------------------------------
+ * Replaces DynamicsPromise::realize<PLATFORM> + ConcreteBlockCollisionO::set
+ * (core/blockDynamicsMap.h:282-350, core/platform/gpu/hip/operator.h:85-97):
+ * table slot `id` (< OLBGFX_MAX_DYNAMICS) holds one dynamics with its
+ * parameters (setParameter<OMEGA>, TRT::MAGIC, FORCE); an unknown kind is
+ * rejected with OLBGFX_ERR_UNSUPPORTED. */
+olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* lattice, uint32_t id,
+                                  const olbgfx_dynamics_desc* dynamics);
+
+/* per-cell dynamics ids, one byte per padded cell in CellID order; cells of
+ * the padding never collide (core/platform/cpu/sisd/operator.h:175-187). */
+olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* lattice, const uint8_t* ids);
+
+/* per-cell field (BlockLattice::defineField, core/blockLattice.h:373-390);
+ * values holds ncells*components entries in CellID order, NULL removes the
+ * field so the uniform dynamics value applies again. */
+olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* lattice, int32_t field,
+                                    int32_t value_type, const void* values);
+
+/* ---- populations -----------------------------------------------------------
+ * iniEquilibrium(indicator, rho, u) (core/blockLattice.hh:202-236,
+ * dynamics/interface.h:111-121): f_bar_i = feq_i(rho, u) on selected cells.
+ * sel: one byte per cell (NULL = all), rho: ncells, u: 3*ncells. */
+olbgfx_status olbgfx_ini_equilibrium(olbgfx_lattice* lattice, const uint8_t* sel,
+                                     int32_t value_type, const void* rho, const void* u);
+
+/* setProcessingContext(Evaluation / Simulation) of the population columns
+ * (core/platform/gpu/hip/column.hh:289-300): copy the populations in logical
+ * order, host[i*ncells + n] = f_bar_i(n).  Synchronise. */
+olbgfx_status olbgfx_get_populations(olbgfx_lattice* lattice, int32_t value_type, void* host);
+olbgfx_status olbgfx_set_populations(olbgfx_lattice* lattice, int32_t value_type, const void* host);
+
+/* ---- the step ---------------------------------------------------------------
+ * SuperLattice::collideAndStream (core/superLattice.hh:663-725) for one block:
+ * collide every core cell, fill the periodic overlap, rotate the rings.
+ * Enqueued on `stream`; does not synchronise. */
+olbgfx_status olbgfx_collide_and_stream(olbgfx_lattice* lattice, uint32_t steps, void* stream);
+
+/* ---- statistics and reductions -------------------------------------------
+ * BlockLattice::setStatisticsEnabled (core/blockLattice.h:239-244).  With statistics on,
+ * each step accumulates, in FP64, the LatticeStatistics aggregate
+ * (core/latticeStatistics.h:50-70): out = {nCells, sum rho, sum uSqr,
+ * max uSqr} of the last step.  get_statistics synchronises. */
+olbgfx_status olbgfx_set_statistics(olbgfx_lattice* lattice, int32_t on);
+olbgfx_status olbgfx_get_statistics(olbgfx_lattice* lattice, double out[4]);
+
+/* FP64 conservation check over all colliding core cells, computed from the
+ * stored populations: out = {nCells, sum (rho-1), sum j_x, sum j_y, sum j_z,
+ * sum |u|^2, max |u|^2, 0}.  Replaces the thrust reductions of
+ * core/platform/gpu/hip/statistics.hh:79-115.  Synchronises. */
+olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* lattice, double out[8], void* stream);
+
+/* density and velocity of every padded cell (Cell::computeRhoU), FP32 or
+ * FP64 host arrays: rho[ncells], u[3*ncells].  Synchronises. */
+olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* lattice, int32_t value_type, void* rho, void* u);
+
+/* ---- launch configuration ---------------------------------------------------
+ * threads per block of the step kernel (64..256, multiple of 64); 0 restores
+ * the default.  Used by the autotuner. */
+olbgfx_status olbgfx_set_launch(olbgfx_lattice* lattice, uint32_t block_threads);
+
+/* block until all work of this lattice is done */
+olbgfx_status olbgfx_synchronize(olbgfx_lattice* lattice);
+
+/* message of the last failed call on this thread ("" if none) */
+const char* olbgfx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OLBGFX_H */

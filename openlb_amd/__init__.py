@@ -1,0 +1,301 @@
+"""openlb_amd -- the gfx950 (MI355X) lattice Boltzmann platform, Python host.
+
+The compute path is ``libolbgfx.so`` (hand-written HIP kernels behind the C ABI
+of ``include/olbgfx.h``).  This module loads it with ctypes and mirrors the
+block-lattice calls of OpenLB's platform interface: dynamics per cell,
+``setParameter``-style dynamics parameters, ``iniEquilibrium``,
+``collideAndStream``, statistics and the evaluation view of the populations.
+
+There is no CPU fallback: if the library is missing or cannot be loaded, the
+import of :class:`Lattice` fails with an error that says so.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libolbgfx.so")
+
+# C ABI constants (include/olbgfx.h)
+STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
+VALUE_F32, VALUE_F64 = 0, 1
+NO_DYNAMICS, BGK, TRT, FORCED_BGK, FORCED_TRT, BOUNCE_BACK, BOUNCE_BACK_VELOCITY = range(7)
+FIELD_VELOCITY, FIELD_RHO, FIELD_FORCE = 0, 1, 2
+MAX_DYNAMICS = 16
+Q = 19
+
+KIND_NAMES = {
+    "NoDynamics": NO_DYNAMICS,
+    "BGKdynamics": BGK,
+    "TRTdynamics": TRT,
+    "ForcedBGKdynamics": FORCED_BGK,
+    "ForcedTRTdynamics": FORCED_TRT,
+    "BounceBack": BOUNCE_BACK,
+    "BounceBackVelocity": BOUNCE_BACK_VELOCITY,
+}
+
+
+class OlbgfxError(RuntimeError):
+    """A C ABI call returned an error status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"olbgfx status {status}: {message}")
+        self.status = status
+
+
+class _LatticeDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("padding", ctypes.c_int32),
+        ("periodic", ctypes.c_int32 * 3),
+        ("storage", ctypes.c_int32),
+    ]
+
+
+class _DynamicsDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("omega", ctypes.c_double),
+        ("magic", ctypes.c_double),
+        ("force", ctypes.c_double * 3),
+        ("wall_velocity", ctypes.c_double * 3),
+        ("wall_rho", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load libolbgfx.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(the gfx950 kernels have no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32
+    sigs = {
+        "olbgfx_lattice_create": [ctypes.POINTER(_LatticeDesc), ctypes.POINTER(vp)],
+        "olbgfx_lattice_destroy": [vp],
+        "olbgfx_lattice_ncells": [vp, ctypes.POINTER(ctypes.c_uint64)],
+        "olbgfx_neighbor_distances": [vp, ctypes.POINTER(ctypes.c_int64)],
+        "olbgfx_ring_shifts": [vp, ctypes.POINTER(ctypes.c_int64)],
+        "olbgfx_set_dynamics": [vp, u32, ctypes.POINTER(_DynamicsDesc)],
+        "olbgfx_set_dynamics_mask": [vp, vp],
+        "olbgfx_set_cell_field": [vp, i32, i32, vp],
+        "olbgfx_ini_equilibrium": [vp, vp, i32, vp, vp],
+        "olbgfx_get_populations": [vp, i32, vp],
+        "olbgfx_set_populations": [vp, i32, vp],
+        "olbgfx_collide_and_stream": [vp, u32, vp],
+        "olbgfx_set_statistics": [vp, i32],
+        "olbgfx_get_statistics": [vp, ctypes.POINTER(ctypes.c_double)],
+        "olbgfx_reduce_moments": [vp, ctypes.POINTER(ctypes.c_double), vp],
+        "olbgfx_compute_rho_u": [vp, i32, vp, vp],
+        "olbgfx_set_launch": [vp, u32],
+        "olbgfx_synchronize": [vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int32
+    lib.olbgfx_last_error.argtypes = []
+    lib.olbgfx_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = library().olbgfx_last_error().decode(errors="replace")
+        raise OlbgfxError(status, msg)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Dynamics:
+    """One dynamics with its parameters (a DynamicsPromise plus setParameter)."""
+
+    kind: int
+    omega: float = 0.0
+    magic: float = 0.0
+    force: tuple = (0.0, 0.0, 0.0)
+    wall_velocity: tuple = (0.0, 0.0, 0.0)
+    wall_rho: float = 1.0
+
+    def _desc(self) -> _DynamicsDesc:
+        d = _DynamicsDesc()
+        d.kind = int(self.kind)
+        d.omega = float(self.omega)
+        d.magic = float(self.magic)
+        for a in range(3):
+            d.force[a] = float(self.force[a])
+            d.wall_velocity[a] = float(self.wall_velocity[a])
+        d.wall_rho = float(self.wall_rho)
+        return d
+
+
+class Lattice:
+    """One D3Q19 block lattice resident in HBM (ConcreteBlockLattice on gfx950).
+
+    Arrays indexed by cell follow OpenLB's CellID order over the padded block
+    (core/blockStructure.h:150-160): CellID = ((x+p)*Py + (y+p))*Pz + (z+p),
+    i.e. a C-order ravel of a (Px, Py, Pz) array.
+    """
+
+    def __init__(self, extent, padding: int = 3, periodic=(False, False, False), storage: str = "f32"):
+        lib = library()
+        nx, ny, nz = (int(v) for v in extent)
+        self.extent = (nx, ny, nz)
+        self.padding = int(padding)
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.storage = storage
+        self.shape = (nx + 2 * self.padding, ny + 2 * self.padding, nz + 2 * self.padding)
+        desc = _LatticeDesc()
+        desc.nx, desc.ny, desc.nz = nx, ny, nz
+        desc.padding = self.padding
+        for a in range(3):
+            desc.periodic[a] = int(self.periodic[a])
+        desc.storage = STORE[storage]
+        h = ctypes.c_void_p()
+        _check(lib.olbgfx_lattice_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_uint64()
+        _check(lib.olbgfx_lattice_ncells(self._h, ctypes.byref(n)))
+        self.ncells = int(n.value)
+        self._value = VALUE_F64 if storage == "f64" else VALUE_F32
+        self._dtype = np.float64 if storage == "f64" else np.float32
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            library().olbgfx_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- structure ----------------------------------------------------------------
+    def cell_id(self, x: int, y: int, z: int) -> int:
+        p = self.padding
+        return ((x + p) * self.shape[1] + (y + p)) * self.shape[2] + (z + p)
+
+    def neighbor_distances(self) -> np.ndarray:
+        out = np.zeros(Q, dtype=np.int64)
+        _check(library().olbgfx_neighbor_distances(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    def ring_shifts(self) -> np.ndarray:
+        out = np.zeros(Q, dtype=np.int64)
+        _check(library().olbgfx_ring_shifts(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
+
+    # -- dynamics -------------------------------------------------------------------
+    def set_dynamics(self, slot: int, dynamics: Dynamics) -> None:
+        d = dynamics._desc()
+        _check(library().olbgfx_set_dynamics(self._h, int(slot), ctypes.byref(d)))
+
+    def set_dynamics_mask(self, ids: np.ndarray) -> None:
+        ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)
+        if ids.size != self.ncells:
+            raise ValueError(f"mask has {ids.size} entries, lattice has {self.ncells} cells")
+        _check(library().olbgfx_set_dynamics_mask(self._h, _ptr(ids)))
+
+    def set_cell_field(self, field: int, values: np.ndarray | None) -> None:
+        if values is None:
+            _check(library().olbgfx_set_cell_field(self._h, field, self._value, None))
+            return
+        comps = 1 if field == FIELD_RHO else 3
+        v = np.ascontiguousarray(values, dtype=self._dtype).reshape(-1)
+        if v.size != comps * self.ncells:
+            raise ValueError("field array has the wrong size")
+        _check(library().olbgfx_set_cell_field(self._h, field, self._value, _ptr(v)))
+
+    # -- populations -------------------------------------------------------------
+    def ini_equilibrium(self, rho: np.ndarray, u: np.ndarray, sel: np.ndarray | None = None) -> None:
+        rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)
+        u = np.ascontiguousarray(u, dtype=self._dtype).reshape(-1)
+        if rho.size != self.ncells or u.size != 3 * self.ncells:
+            raise ValueError("rho/u arrays have the wrong size")
+        s = None if sel is None else np.ascontiguousarray(sel, dtype=np.uint8).reshape(-1)
+        _check(library().olbgfx_ini_equilibrium(self._h, _ptr(s), self._value, _ptr(rho), _ptr(u)))
+
+    def get_populations(self) -> np.ndarray:
+        """Populations in logical order, shape (19, ncells), shifted by -t_i."""
+        out = np.empty((Q, self.ncells), dtype=self._dtype)
+        _check(library().olbgfx_get_populations(self._h, self._value, _ptr(out)))
+        return out
+
+    def set_populations(self, f: np.ndarray) -> None:
+        f = np.ascontiguousarray(f, dtype=self._dtype)
+        if f.shape != (Q, self.ncells):
+            raise ValueError("populations must have shape (19, ncells)")
+        _check(library().olbgfx_set_populations(self._h, self._value, _ptr(f)))
+
+    # -- the step ------------------------------------------------------------------
+    def collide_and_stream(self, steps: int = 1, stream: int | None = None) -> None:
+        _check(library().olbgfx_collide_and_stream(self._h, int(steps), stream))
+
+    collideAndStream = collide_and_stream
+
+    def synchronize(self) -> None:
+        _check(library().olbgfx_synchronize(self._h))
+
+    def set_launch(self, block_threads: int) -> None:
+        _check(library().olbgfx_set_launch(self._h, int(block_threads)))
+
+    # -- statistics -----------------------------------------------------------------
+    def set_statistics(self, on: bool) -> None:
+        _check(library().olbgfx_set_statistics(self._h, int(bool(on))))
+
+    def get_statistics(self) -> dict:
+        out = (ctypes.c_double * 4)()
+        _check(library().olbgfx_get_statistics(self._h, out))
+        n = out[0]
+        return {
+            "nCells": n,
+            "sumRho": out[1],
+            "sumUSqr": out[2],
+            "maxUSqr": out[3],
+            "averageRho": out[1] / n if n else 0.0,
+            "averageEnergy": out[2] / n if n else 0.0,
+            "maxU": float(np.sqrt(out[3])),
+        }
+
+    def reduce_moments(self, stream: int | None = None) -> dict:
+        out = (ctypes.c_double * 8)()
+        _check(library().olbgfx_reduce_moments(self._h, out, stream))
+        return {
+            "nCells": out[0],
+            "mass": out[1],
+            "momentum": (out[2], out[3], out[4]),
+            "sumUSqr": out[5],
+            "maxUSqr": out[6],
+        }
+
+    def compute_rho_u(self):
+        rho = np.empty(self.ncells, dtype=self._dtype)
+        u = np.empty(3 * self.ncells, dtype=self._dtype)
+        _check(library().olbgfx_compute_rho_u(self._h, self._value, _ptr(rho), _ptr(u)))
+        return rho, u.reshape(self.ncells, 3)

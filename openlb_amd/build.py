@@ -1,0 +1,70 @@
+"""Build recipes for the gfx950 platform library and the CPU checker.
+
+``build_library`` compiles ``openlb_amd/csrc/olbgfx.hip`` with hipcc for
+gfx950 into ``openlb_amd/libolbgfx.so`` (in-tree, so it travels with the
+source tree).  ``build_oracle`` compiles the C restatement in ``oracle/`` and,
+when the OpenLB sources are present, the reference driver in ``oracle/_ref``;
+both are checkers used only by the tests and the CPU baseline.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "openlb_amd")
+LIB = os.path.join(PKG, "libolbgfx.so")
+HIP_SOURCES = [os.path.join(PKG, "csrc", "olbgfx.hip")]
+
+HIPCC_FLAGS = [
+    "-O3",
+    "--offload-arch=gfx950",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # packed FP32 instructions stay off (compiler-generated v_pk_* f32 forms)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+    "-Wno-unused-value",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
+
+
+def _run(cmd: list[str], cwd: str | None = None) -> None:
+    proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+
+
+def build_library() -> str:
+    """Compile the HIP kernels and the C ABI into openlb_amd/libolbgfx.so."""
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *HIP_SOURCES, "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    """Compile the CPU restatement (and the reference checker if possible)."""
+    _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
+    if with_reference and os.path.isdir("/root/reference/src"):
+        _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
+
+
+def build_all() -> None:
+    build_library()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,286 @@
+"""Case set-up for the D3Q19 workloads: geometry (material numbers per cell),
+dynamics per material and initial populations.
+
+A ``Case`` is plain data; ``Case.apply(lattice)`` performs the calls an OpenLB
+``prepareLattice`` / ``setInitialValues`` makes (``dynamics::set`` per
+material, ``setParameter<OMEGA>``, ``defineField``, ``iniEquilibrium``) on
+anything with the block-lattice methods of :class:`openlb_amd.Lattice`.
+
+All per-cell arrays are over the padded block in CellID order, i.e. C-order
+arrays of shape (nx+2p, ny+2p, nz+2p) (core/blockStructure.h:150-160).
+Material 0 marks cells outside the domain (padding), as in
+SuperGeometry (geometry/blockGeometry.h:59).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FORCED_TRT, NO_DYNAMICS, Dynamics
+
+__all__ = [
+    "Case",
+    "relaxation_frequency",
+    "tgv",
+    "box_cavity",
+    "porous_channel",
+    "sphere_pack",
+]
+
+
+def relaxation_frequency(nu: float) -> float:
+    """omega = 1 / (3 nu + 1/2) (core/unitConverter.h:882 with cs^2 = 1/3)."""
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+@dataclass
+class Case:
+    name: str
+    extent: tuple
+    padding: int
+    periodic: tuple
+    materials: np.ndarray                       # uint8, padded shape
+    dynamics: dict = field(default_factory=dict)  # material -> Dynamics
+    rho: np.ndarray | None = None               # padded, initial density
+    u: np.ndarray | None = None                 # padded x 3, initial velocity
+    init_materials: tuple = ()                  # materials that get iniEquilibrium
+    velocity_field: np.ndarray | None = None    # optional per-cell VELOCITY
+    info: dict = field(default_factory=dict)
+
+    @property
+    def shape(self):
+        return self.materials.shape
+
+    @property
+    def ncells(self) -> int:
+        return int(self.materials.size)
+
+    def n_voxels(self) -> int:
+        """cells with material != 0 (SuperGeometryStatistics::getNvoxel,
+        geometry/superGeometryStatistics3D.hh:319-328)"""
+        return int(np.count_nonzero(self.materials))
+
+    def dynamics_slots(self):
+        """material -> dynamics table slot (slot 0 stays NoDynamics)."""
+        slots = {}
+        table = [Dynamics(NO_DYNAMICS)]
+        for mat in sorted(self.dynamics):
+            table.append(self.dynamics[mat])
+            slots[mat] = len(table) - 1
+        return slots, table
+
+    def dynamics_mask(self) -> np.ndarray:
+        slots, _ = self.dynamics_slots()
+        lut = np.zeros(256, dtype=np.uint8)
+        for mat, slot in slots.items():
+            lut[mat] = slot
+        return lut[self.materials.reshape(-1)]
+
+    def apply(self, lattice) -> None:
+        slots, table = self.dynamics_slots()
+        for slot, dyn in enumerate(table):
+            lattice.set_dynamics(slot, dyn)
+        lattice.set_dynamics_mask(self.dynamics_mask())
+        if self.velocity_field is not None:
+            lattice.set_cell_field(FIELD_VELOCITY, self.velocity_field)
+        if self.rho is not None:
+            sel = np.isin(self.materials.reshape(-1), self.init_materials).astype(np.uint8)
+            lattice.ini_equilibrium(self.rho.reshape(-1), self.u.reshape(-1, 3), sel)
+
+    def core_mask(self) -> np.ndarray:
+        p = self.padding
+        m = np.zeros(self.shape, dtype=bool)
+        m[p:-p, p:-p, p:-p] = True
+        return m
+
+
+def _padded(extent, padding):
+    return tuple(int(n) + 2 * padding for n in extent)
+
+
+def tgv(n: int, u0: float = 0.04, tau: float = 0.56, padding: int = 3,
+        dtype=np.float64, with_pressure: bool = True) -> Case:
+    """Fully periodic Taylor-Green vortex on an n^3 box.
+
+    u = u0 (sin x cos y cos z, -cos x sin y cos z, 0) with x = 2 pi i / n, and
+    the matching pressure p = (u0^2 / 16)(cos 2x + cos 2y)(cos 2z + 2),
+    rho = 1 + 3 p (cs^2 = 1/3).  BGK with omega = 1/tau everywhere
+    (examples/turbulence/tgv3d/tgv3d.cpp:68-92 with the DNS bulk dynamics).
+    """
+    extent = (n, n, n)
+    shape = _padded(extent, padding)
+    mats = np.zeros(shape, dtype=np.uint8)
+    mats[padding:-padding, padding:-padding, padding:-padding] = 1
+    k = 2.0 * np.pi / n
+    idx = (np.arange(shape[0]) - padding).astype(np.float64)
+    x = (k * idx)
+    sx, cx = np.sin(x).astype(dtype), np.cos(x).astype(dtype)
+    c2 = np.cos(2 * x).astype(dtype)
+    u = np.empty(shape + (3,), dtype=dtype)
+    u[..., 0] = u0 * sx[:, None, None] * cx[None, :, None] * cx[None, None, :]
+    u[..., 1] = -u0 * cx[:, None, None] * sx[None, :, None] * cx[None, None, :]
+    u[..., 2] = 0
+    if with_pressure:
+        p = (u0 * u0 / 16.0) * (c2[:, None, None] + c2[None, :, None]) * (c2[None, None, :] + 2.0)
+        rho = (1.0 + 3.0 * p).astype(dtype)
+    else:
+        rho = np.ones(shape, dtype=dtype)
+    omega = 1.0 / tau
+    return Case(
+        name=f"tgv{n}",
+        extent=extent,
+        padding=padding,
+        periodic=(True, True, True),
+        materials=mats,
+        dynamics={1: Dynamics(BGK, omega=omega)},
+        rho=rho,
+        u=u,
+        init_materials=(1,),
+        info={"u0": u0, "tau": tau, "nu": (tau - 0.5) / 3.0, "k": k},
+    )
+
+
+def box_cavity(n: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,
+               dtype=np.float64) -> Case:
+    """Lid-driven cavity on an n^3 core: the outer layer is wall (material 2,
+    BounceBack), the y = n-1 face without its rim is the lid (material 3,
+    BounceBackVelocity with u = (u_lid, 0, 0)), the rest is fluid (material 1,
+    BGK).  nu = u_lid * n / re; starts at rest with rho = 1.
+    """
+    extent = (n, n, n)
+    shape = _padded(extent, padding)
+    p = padding
+    mats = np.zeros(shape, dtype=np.uint8)
+    core = mats[p:-p, p:-p, p:-p]
+    core[...] = 2
+    core[1:-1, 1:-1, 1:-1] = 1
+    core[1:-1, n - 1, 1:-1] = 3
+    nu = u_lid * n / re
+    omega = relaxation_frequency(nu)
+    rho = np.ones(shape, dtype=dtype)
+    u = np.zeros(shape + (3,), dtype=dtype)
+    return Case(
+        name=f"cavity{n}",
+        extent=extent,
+        padding=padding,
+        periodic=(False, False, False),
+        materials=mats,
+        dynamics={
+            1: Dynamics(BGK, omega=omega),
+            2: Dynamics(BOUNCE_BACK),
+            3: Dynamics(BOUNCE_BACK_VELOCITY, wall_velocity=(u_lid, 0.0, 0.0), wall_rho=1.0),
+        },
+        rho=rho,
+        u=u,
+        init_materials=(1, 2, 3),
+        info={"nu": nu, "omega": omega, "u_lid": u_lid},
+    )
+
+
+class _SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014), a portable seeded generator so the
+    sphere positions are the same everywhere."""
+
+    def __init__(self, seed: int):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def sphere_pack(extent, radius: float, porosity: float, seed: int, max_tries: int = 200000):
+    """Non-overlapping spheres (centres in lattice units) placed by random
+    sequential addition until the solid fraction reaches 1 - porosity.
+    Spheres may cross the periodic faces; overlap is tested with the minimum
+    image convention."""
+    rng = _SplitMix64(seed)
+    ext = np.asarray(extent, dtype=np.float64)
+    vol = float(np.prod(ext))
+    target = (1.0 - porosity) * vol
+    sphere_vol = 4.0 / 3.0 * np.pi * radius ** 3
+    centres: list = []
+    tries = 0
+    while len(centres) * sphere_vol < target and tries < max_tries:
+        tries += 1
+        c = np.array([rng.uniform() * ext[a] for a in range(3)])
+        ok = True
+        for d in centres:
+            diff = np.abs(c - d)
+            diff = np.minimum(diff, ext - diff)
+            if float(diff @ diff) < (2.0 * radius) ** 2:
+                ok = False
+                break
+        if ok:
+            centres.append(c)
+    return np.array(centres).reshape(-1, 3)
+
+
+def porous_materials(extent, centres, radius: float, padding: int) -> np.ndarray:
+    """material 1 fluid, 2 solid (cell centre within radius of a sphere centre,
+    periodic images included), 0 in the padding."""
+    shape = _padded(extent, padding)
+    mats = np.zeros(shape, dtype=np.uint8)
+    nx, ny, nz = extent
+    if min(extent) < 2 * radius + 2:
+        raise ValueError("every extent must exceed the sphere diameter")
+    core = np.ones((nx, ny, nz), dtype=np.uint8)
+    r2 = radius * radius
+    for c in centres:
+        lo = np.floor(c - radius).astype(int)
+        hi = np.ceil(c + radius).astype(int)
+        xs = np.arange(lo[0], hi[0] + 1)
+        ys = np.arange(lo[1], hi[1] + 1)
+        zs = np.arange(lo[2], hi[2] + 1)
+        dx = (xs - c[0]) ** 2
+        dy = (ys - c[1]) ** 2
+        dz = (zs - c[2]) ** 2
+        inside = (dx[:, None, None] + dy[None, :, None] + dz[None, None, :]) <= r2
+        ix = np.mod(xs, nx)
+        iy = np.mod(ys, ny)
+        iz = np.mod(zs, nz)
+        sub = core[np.ix_(ix, iy, iz)]
+        sub[inside] = 2
+        core[np.ix_(ix, iy, iz)] = sub
+    p = padding
+    mats[p:-p, p:-p, p:-p] = core
+    return mats
+
+
+def porous_channel(extent=(384, 128, 128), radius: float = 6.0, porosity: float = 0.7,
+                   seed: int = 20261202, force: float = 1e-6, tau: float = 0.8,
+                   magic: float = 3.0 / 16.0, padding: int = 3, dtype=np.float64) -> Case:
+    """Body-force driven flow through a periodic sphere pack: TRT with
+    Lambda = magic and Guo forcing (ForcedTRTdynamics) in the fluid, BounceBack
+    in the solid, periodic in x, y and z, starting at rest."""
+    centres = sphere_pack(extent, radius, porosity, seed)
+    mats = porous_materials(extent, centres, radius, padding)
+    shape = mats.shape
+    omega = 1.0 / tau
+    rho = np.ones(shape, dtype=dtype)
+    u = np.zeros(shape + (3,), dtype=dtype)
+    solid = int(np.count_nonzero(mats == 2))
+    return Case(
+        name="porous",
+        extent=tuple(extent),
+        padding=padding,
+        periodic=(True, True, True),
+        materials=mats,
+        dynamics={
+            1: Dynamics(FORCED_TRT, omega=omega, magic=magic, force=(force, 0.0, 0.0)),
+            2: Dynamics(BOUNCE_BACK),
+        },
+        rho=rho,
+        u=u,
+        init_materials=(1, 2),
+        info={"spheres": centres, "porosity": 1.0 - solid / float(np.prod(extent)),
+              "tau": tau, "magic": magic, "force": force},
+    )
+

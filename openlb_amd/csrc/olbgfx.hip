@@ -1,0 +1,1504 @@
+// olbgfx.hip -- gfx950 (MI355X) D3Q19 lattice Boltzmann platform behind the
+// C ABI declared in include/olbgfx.h.
+//
+// Storage: 19 structure-of-arrays rings, one per population, in one device
+// allocation (ring stride rounded to 256 B).  Population i of logical cell n
+// lives at ring slot (n + s_i) mod ncells, the periodic-shift (PS) pattern of
+// OpenLB (core/platform/cpu/sisd/column.h:129-204): collision is in place and
+// streaming only changes s_i -= d_i, so one step moves exactly 2*19 values per
+// cell.  Unlike the reference, whose host rotates 19 pointers and copies them
+// to the device every step (core/blockLattice.hh:316-338), the offsets here
+// are kernel arguments, or live in device memory so a captured graph can be
+// replayed.
+//
+// The step is one kernel: every core cell loads its 19 populations (coalesced
+// along z), applies its dynamics (BGK, TRT, Guo-forced BGK/TRT, BounceBack,
+// BounceBackVelocity, selected by a one-byte id per cell or uniformly), stores
+// them back, and -- on periodic boundary planes -- also writes the populations
+// that cross the boundary into the width-1 ghost layer.  That last part
+// replaces the reference's PostCollide overlap copy
+// (core/platform/gpu/hip/communicator.hh:529-585): only the populations a core
+// cell will pull are written.  Optionally the same pass reduces the lattice
+// statistics in FP64 with wavefront shuffles.
+//
+// The roofline of the step is HBM bandwidth: 2*19*sizeof(storage) bytes per
+// cell update against ~400 flops, far below the ridge point, so no MFMA.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../../include/olbgfx.h"
+
+namespace {
+
+constexpr int Q = 19;
+// largest workgroup of the step kernel; more waves per block would cap the
+// registers per lane below what the D3Q19 collision needs without spilling
+constexpr uint32_t kMaxBlock = 256;
+
+// D3Q19 velocity set, descriptor/definition/common.h:166-199.
+// Index 0 rests, 1..9 are the negative half, i+9 is the opposite of i.
+__host__ __device__ constexpr int CX(int i)
+{
+  constexpr int v[Q] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+  return v[i];
+}
+__host__ __device__ constexpr int CY(int i)
+{
+  constexpr int v[Q] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+  return v[i];
+}
+__host__ __device__ constexpr int CZ(int i)
+{
+  constexpr int v[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+  return v[i];
+}
+__host__ __device__ constexpr int TDEN(int i)
+{
+  return i == 0 ? 3 : ((i % 9 >= 1 && i % 9 <= 3) ? 18 : 36);
+}
+
+// ---------------------------------------------------------------------------
+// storage types
+
+struct bf16_t {
+  uint16_t bits;
+};
+
+template <typename S>
+struct Storage;
+
+template <>
+struct Storage<float> {
+  using A = float;
+  static __device__ __forceinline__ float load(const float* p) { return *p; }
+  static __device__ __forceinline__ float pack(float v) { return v; }
+  static __device__ __forceinline__ float unpack(float v) { return v; }
+};
+
+template <>
+struct Storage<double> {
+  using A = double;
+  static __device__ __forceinline__ double load(const double* p) { return *p; }
+  static __device__ __forceinline__ double pack(double v) { return v; }
+  static __device__ __forceinline__ double unpack(double v) { return v; }
+};
+
+template <>
+struct Storage<_Float16> {
+  using A = float;
+  // v_cvt_f16_f32 rounds to nearest even and keeps FP16 subnormals
+  static __device__ __forceinline__ float load(const _Float16* p) { return (float)*p; }
+  static __device__ __forceinline__ _Float16 pack(float v) { return (_Float16)v; }
+  static __device__ __forceinline__ float unpack(_Float16 v) { return (float)v; }
+};
+
+template <>
+struct Storage<bf16_t> {
+  using A = float;
+  static __device__ __forceinline__ float unpack(bf16_t b)
+  {
+    return __uint_as_float((uint32_t)b.bits << 16);
+  }
+  static __device__ __forceinline__ float load(const bf16_t* p) { return unpack(*p); }
+  // round to nearest even; NaN stays a quiet NaN
+  static __device__ __forceinline__ bf16_t pack(float v)
+  {
+    uint32_t x = __float_as_uint(v);
+    if ((x & 0x7fffffffu) > 0x7f800000u) {
+      return bf16_t{(uint16_t)((x >> 16) | 0x40u)};
+    }
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return bf16_t{(uint16_t)(x >> 16)};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// kernel arguments
+
+template <typename A>
+struct DynEntry {
+  int32_t kind;
+  A omega;
+  A omega2;      // TRT odd relaxation, formed on the host like collision.h:408
+  A force[3];
+  A wall_u[3];
+  A wall_rho;
+};
+
+template <typename A>
+struct StepArgs {
+  void* pop;                 // ring 0; ring i starts at pop + i*stride elements
+  uint64_t stride;
+  uint32_t count;            // ring length = padded cells
+  uint32_t shift[Q];         // used when shift_dev is null
+  const uint32_t* shift_dev; // device copy of the offsets (graph replay)
+  const uint8_t* mask;       // dynamics id per padded cell
+  int32_t uniform_id;        // >= 0: every core cell has this id, mask unused
+  int32_t nx, ny, nz, pad;
+  uint32_t py, pz;           // padded extent in y and z
+  uint32_t nyz;              // ny*nz
+  uint64_t magic_nyz;        // ceil(2^64 / nyz), 0 if nyz == 1
+  uint64_t magic_nz;         // ceil(2^64 / nz),  0 if nz == 1
+  uint32_t ncore;            // nx*ny*nz
+  uint32_t nblocks;          // blocks that carry cells
+  uint32_t blocks_per_xcd;   // ceil(nblocks / 8)
+  int32_t periodic;          // bit a set: axis a periodic
+  const A* cell_u;           // optional VELOCITY field (3 per cell)
+  const A* cell_rho;         // optional RHO field
+  const A* cell_force;       // optional FORCE field (3 per cell)
+  double* stats;             // 4 doubles per block when statistics are on
+  DynEntry<A> table[OLBGFX_MAX_DYNAMICS];
+};
+
+// n / d for 32-bit n, with m = ceil(2^64 / d) (exact for every 32-bit n)
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint64_t m, uint32_t d)
+{
+  if (m == 0) {
+    return n / d;
+  }
+  const uint64_t lo = (uint64_t)(uint32_t)m * n;
+  const uint64_t hi = (m >> 32) * n;
+  return (uint32_t)((hi + (lo >> 32)) >> 32);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v += __shfl_xor(v, o, 64);
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v = fmax(v, __shfl_xor(v, o, 64));
+  }
+  return v;
+}
+
+// block-wide FP64 reduction of four values into out[0..3]
+__device__ void block_reduce4(double c, double r, double e, double m, double* out, bool write)
+{
+  __shared__ double part[16][4];
+  c = wave_sum(c);
+  r = wave_sum(r);
+  e = wave_sum(e);
+  m = wave_max(m);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    part[wave][0] = c;
+    part[wave][1] = r;
+    part[wave][2] = e;
+    part[wave][3] = m;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const int nw = blockDim.x >> 6;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (lane < nw) {
+      a0 = part[lane][0];
+      a1 = part[lane][1];
+      a2 = part[lane][2];
+      a3 = part[lane][3];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    a3 = wave_max(a3);
+    if (lane == 0 && write) {
+      out[0] = a0;
+      out[1] = a1;
+      out[2] = a2;
+      out[3] = a3;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-cell dynamics on 19 registers
+
+template <typename A>
+__device__ __forceinline__ void moments(const A (&f)[Q], A& drho, A& jx, A& jy, A& jz)
+{
+  // pairwise sums keep the dependency chains short
+  A s = (f[0] + f[1]) + (f[2] + f[3]);
+  s += (f[4] + f[5]) + (f[6] + f[7]);
+  s += (f[8] + f[9]) + (f[10] + f[11]);
+  s += (f[12] + f[13]) + (f[14] + f[15]);
+  s += (f[16] + f[17]) + f[18];
+  drho = s;
+  jx = ((f[10] - f[1]) + (f[13] - f[4])) + ((f[14] - f[5]) + (f[15] - f[6])) + (f[16] - f[7]);
+  jy = ((f[11] - f[2]) + (f[13] - f[4])) + ((f[5] - f[14]) + (f[17] - f[8])) + (f[18] - f[9]);
+  jz = ((f[12] - f[3]) + (f[15] - f[6])) + ((f[7] - f[16]) + (f[17] - f[8])) + (f[9] - f[18]);
+}
+
+template <int I, typename A>
+__device__ __forceinline__ A cdot(A x, A y, A z)
+{
+  A r = A(0);
+  bool first = true;
+  if constexpr (CX(I) != 0) {
+    r = CX(I) > 0 ? x : -x;
+    first = false;
+  }
+  if constexpr (CY(I) != 0) {
+    r = first ? (CY(I) > 0 ? y : -y) : (CY(I) > 0 ? r + y : r - y);
+    first = false;
+  }
+  if constexpr (CZ(I) != 0) {
+    r = first ? (CZ(I) > 0 ? z : -z) : (CZ(I) > 0 ? r + z : r - z);
+  }
+  return r;
+}
+
+template <int I, typename A>
+__device__ __forceinline__ constexpr A weight()
+{
+  return A(1) / A(TDEN(I));
+}
+
+// BGK (collision.h:57-80) or TRT (collision.h:381-419) relaxation towards the
+// shifted second-order equilibrium (lbm.h:51-64), written in even/odd pairs:
+// feq_{i,opp} = sym +- asym with
+//   sym  = t_i (drho + rho (4.5 (c.u)^2 - 1.5 u^2)),  asym = 3 t_i rho (c.u).
+template <bool TRT, int P, typename A>
+__device__ __forceinline__ void relax_pair(A (&f)[Q], A drho, A rho, A ux, A uy, A uz,
+                                           A k15, A om, A om2)
+{
+  const A t = weight<P, A>();
+  const A cu = cdot<P>(ux, uy, uz);
+  const A sym = t * (drho + rho * (A(4.5) * cu * cu - k15));
+  const A asym = t * A(3) * rho * cu;
+  const A fi = f[P];
+  const A fo = f[P - 9];
+  if constexpr (!TRT) {
+    f[P] = fi + om * ((sym + asym) - fi);
+    f[P - 9] = fo + om * ((sym - asym) - fo);
+  } else {
+    const A dp = om * (A(0.5) * (fi + fo) - sym);
+    const A dm = om2 * (A(0.5) * (fi - fo) - asym);
+    f[P] = (fi - dp) - dm;
+    f[P - 9] = (fo - dp) + dm;
+  }
+}
+
+template <bool TRT, typename A>
+__device__ __forceinline__ void relax(A (&f)[Q], A drho, A rho, A ux, A uy, A uz, A om, A om2)
+{
+  const A k15 = A(1.5) * (ux * ux + uy * uy + uz * uz);
+  const A feq0 = weight<0, A>() * (drho - rho * k15);
+  f[0] = f[0] + om * (feq0 - f[0]);
+  relax_pair<TRT, 10>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 11>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 12>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 13>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 14>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 15>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 16>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 17>(f, drho, rho, ux, uy, uz, k15, om, om2);
+  relax_pair<TRT, 18>(f, drho, rho, ux, uy, uz, k15, om, om2);
+}
+
+// Guo force term after the collision (lbm.h:567-588):
+// f_i += rho (1 - omega/2) t_i [3 (c_i - u).F + 9 (c_i.u)(c_i.F)]
+template <int P, typename A>
+__device__ __forceinline__ void force_pair(A (&f)[Q], A k, A ux, A uy, A uz, A fx, A fy, A fz, A uf)
+{
+  const A t = weight<P, A>();
+  const A cu = cdot<P>(ux, uy, uz);
+  const A cf = cdot<P>(fx, fy, fz);
+  const A sym = k * t * (A(9) * cu * cf - A(3) * uf);
+  const A asym = k * t * A(3) * cf;
+  f[P] += sym + asym;
+  f[P - 9] += sym - asym;
+}
+
+template <typename A>
+__device__ __forceinline__ void add_force(A (&f)[Q], A rho, A om, A ux, A uy, A uz, A fx, A fy, A fz)
+{
+  const A k = rho * (A(1) - om * A(0.5));
+  const A uf = ux * fx + uy * fy + uz * fz;
+  f[0] += k * weight<0, A>() * (A(-3) * uf);
+  force_pair<10>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<11>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<12>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<13>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<14>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<15>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<16>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<17>(f, k, ux, uy, uz, fx, fy, fz, uf);
+  force_pair<18>(f, k, ux, uy, uz, fx, fy, fz, uf);
+}
+
+// Revert (collision.h:465-484) and the Nguyen-Ladd moving-wall correction
+// f_i += 6 rho_w t_i (c_i . u_w) (collision.h:494-520)
+template <int P, typename A>
+__device__ __forceinline__ void revert_pair(A (&f)[Q], bool moving, A rw, A wx, A wy, A wz)
+{
+  const A a = f[P];
+  f[P] = f[P - 9];
+  f[P - 9] = a;
+  if (moving) {
+    const A corr = A(6) * rw * weight<P, A>() * cdot<P>(wx, wy, wz);
+    f[P] += corr;
+    f[P - 9] -= corr;
+  }
+}
+
+template <typename A>
+__device__ __forceinline__ void revert(A (&f)[Q], bool moving, A rw, A wx, A wy, A wz)
+{
+  revert_pair<10>(f, moving, rw, wx, wy, wz);
+  revert_pair<11>(f, moving, rw, wx, wy, wz);
+  revert_pair<12>(f, moving, rw, wx, wy, wz);
+  revert_pair<13>(f, moving, rw, wx, wy, wz);
+  revert_pair<14>(f, moving, rw, wx, wy, wz);
+  revert_pair<15>(f, moving, rw, wx, wy, wz);
+  revert_pair<16>(f, moving, rw, wx, wy, wz);
+  revert_pair<17>(f, moving, rw, wx, wy, wz);
+  revert_pair<18>(f, moving, rw, wx, wy, wz);
+}
+
+// apply dynamics KIND to one cell; returns true if it yields a statistic
+// (CellStatistic with rho != -1, dynamics/interface.h:44-52)
+template <int KIND, typename A>
+__device__ __forceinline__ bool apply_kind(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
+                                           const StepArgs<A>& a, A& st_rho, A& st_usqr)
+{
+  if constexpr (KIND >= OLBGFX_BGK && KIND <= OLBGFX_FORCED_TRT) {
+    constexpr bool forced = KIND == OLBGFX_FORCED_BGK || KIND == OLBGFX_FORCED_TRT;
+    constexpr bool trt = KIND == OLBGFX_TRT || KIND == OLBGFX_FORCED_TRT;
+    A drho, jx, jy, jz;
+    moments(f, drho, jx, jy, jz);
+    const A rho = drho + A(1);
+    A ux = jx / rho;
+    A uy = jy / rho;
+    A uz = jz / rho;
+    A fx = A(0), fy = A(0), fz = A(0);
+    if constexpr (forced) {
+      if (a.cell_force) {
+        fx = a.cell_force[3 * (size_t)n + 0];
+        fy = a.cell_force[3 * (size_t)n + 1];
+        fz = a.cell_force[3 * (size_t)n + 2];
+      } else {
+        fx = e.force[0];
+        fy = e.force[1];
+        fz = e.force[2];
+      }
+      // momenta::ForcedMomentum::computeU shifts u by F/2 (elements.h:1600-1608)
+      ux += fx * A(0.5);
+      uy += fy * A(0.5);
+      uz += fz * A(0.5);
+    }
+    relax<trt>(f, drho, rho, ux, uy, uz, e.omega, e.omega2);
+    if constexpr (forced) {
+      add_force(f, rho, e.omega, ux, uy, uz, fx, fy, fz);
+    }
+    st_rho = rho;
+    st_usqr = ux * ux + uy * uy + uz * uz;
+    return true;
+  } else if constexpr (KIND == OLBGFX_BOUNCE_BACK) {
+    revert(f, false, A(0), A(0), A(0), A(0));
+    return false;
+  } else if constexpr (KIND == OLBGFX_BOUNCE_BACK_VELOCITY) {
+    const A rw = a.cell_rho ? a.cell_rho[n] : e.wall_rho;
+    A wx = e.wall_u[0], wy = e.wall_u[1], wz = e.wall_u[2];
+    if (a.cell_u) {
+      wx = a.cell_u[3 * (size_t)n + 0];
+      wy = a.cell_u[3 * (size_t)n + 1];
+      wz = a.cell_u[3 * (size_t)n + 2];
+    }
+    revert(f, true, rw, wx, wy, wz);
+    st_rho = rw;
+    st_usqr = wx * wx + wy * wy + wz * wz;
+    return true;
+  } else {
+    return false;
+  }
+}
+
+template <typename A>
+__device__ __forceinline__ bool apply_any(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
+                                          const StepArgs<A>& a, A& st_rho, A& st_usqr)
+{
+  switch (e.kind) {
+  case OLBGFX_BGK:
+    return apply_kind<OLBGFX_BGK>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_TRT:
+    return apply_kind<OLBGFX_TRT>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_FORCED_BGK:
+    return apply_kind<OLBGFX_FORCED_BGK>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_FORCED_TRT:
+    return apply_kind<OLBGFX_FORCED_TRT>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_BOUNCE_BACK:
+    return apply_kind<OLBGFX_BOUNCE_BACK>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_BOUNCE_BACK_VELOCITY:
+    return apply_kind<OLBGFX_BOUNCE_BACK_VELOCITY>(f, e, n, a, st_rho, st_usqr);
+  default:
+    return false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ring addressing: ring i is a uniform base pointer and the slot a 32-bit
+// byte offset, so loads and stores take the scalar-base + vector-offset form
+
+__device__ __forceinline__ uint32_t ring_slot(uint32_t n, uint32_t shift, uint32_t count)
+{
+  const uint32_t s = n + shift;
+  return s >= count ? s - count : s;
+}
+
+template <typename S>
+__device__ __forceinline__ S* ring_at(void* pop, uint64_t stride, int i, uint32_t slot)
+{
+  char* base = (char*)pop + (uint64_t)i * stride * sizeof(S);
+  return (S*)(base + slot * (uint32_t)sizeof(S));
+}
+
+// ---------------------------------------------------------------------------
+// ghost writes of the periodic overlap
+
+// population P of a boundary cell n goes to every ghost image g = n + sum of
+// offsets along the axes where it leaves the core through a periodic face
+template <int P, typename S>
+__device__ __forceinline__ void ghost_pop(void* pop, uint64_t stride, uint32_t count,
+                                          uint32_t shift, int64_t n, const bool (&lo)[3],
+                                          const bool (&hi)[3], const int64_t (&off)[3], S v)
+{
+  constexpr int c[3] = {CX(P), CY(P), CZ(P)};
+  bool el[3];
+  int64_t o[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    el[ax] = (c[ax] > 0 && hi[ax]) || (c[ax] < 0 && lo[ax]);
+    o[ax] = c[ax] > 0 ? -off[ax] : off[ax];
+  }
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (c[ax] != 0 && el[ax]) {
+      *ring_at<S>(pop, stride, P, ring_slot((uint32_t)(n + o[ax]), shift, count)) = v;
+    }
+  }
+  // edges (D3Q19 has at most two non-zero components)
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+#pragma unroll
+    for (int bx = ax + 1; bx < 3; ++bx) {
+      if (c[ax] != 0 && c[bx] != 0 && el[ax] && el[bx]) {
+        *ring_at<S>(pop, stride, P, ring_slot((uint32_t)(n + o[ax] + o[bx]), shift, count)) = v;
+      }
+    }
+  }
+}
+
+template <typename S, typename A, int... P>
+__device__ __forceinline__ void ghost_all(void* pop, uint64_t stride, uint32_t count,
+                                          const uint32_t (&sh)[Q], int64_t n, const bool (&lo)[3],
+                                          const bool (&hi)[3], const int64_t (&off)[3],
+                                          const A (&f)[Q], std::integer_sequence<int, P...>)
+{
+  (ghost_pop<P + 1, S>(pop, stride, count, sh[P + 1], n, lo, hi, off,
+                       Storage<S>::pack(f[P + 1])),
+   ...);
+}
+
+// ---------------------------------------------------------------------------
+// the step kernel
+//
+// MODE < 0: every cell reads its dynamics id from the mask;
+// MODE = k: every core cell has the dynamics of table slot uniform_id, whose
+//           kind is k, so the mask is not read and only that path is compiled.
+
+template <typename S, int MODE, bool STATS>
+__global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename Storage<S>::A> a)
+{
+  using A = typename Storage<S>::A;
+  using St = Storage<S>;
+
+  // XCD-aware order: the hardware deals consecutive workgroups round-robin to
+  // the 8 XCDs; map them so each XCD sweeps one contiguous part of the block
+  // and the cache lines shared by neighbouring segments stay in one L2.
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  const uint32_t lin = lb * blockDim.x + threadIdx.x;
+  const bool in_grid = lb < a.nblocks;
+  const bool active = in_grid && lin < a.ncore;
+
+  uint32_t sh[Q];
+  if (a.shift_dev) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) sh[i] = a.shift_dev[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) sh[i] = a.shift[i];
+  }
+
+  A st_rho = A(0), st_usqr = A(0);
+  bool counted = false;
+
+  if (active) {
+    const uint32_t x = fast_div(lin, a.magic_nyz, a.nyz);
+    const uint32_t r = lin - x * a.nyz;
+    const uint32_t y = fast_div(r, a.magic_nz, (uint32_t)a.nz);
+    const uint32_t z = r - y * (uint32_t)a.nz;
+    const uint32_t n = ((x + a.pad) * a.py + (y + a.pad)) * a.pz + (z + a.pad);
+
+    int kind;
+    int id;
+    if constexpr (MODE >= 0) {
+      id = a.uniform_id;
+      kind = MODE;
+    } else {
+      id = (int)a.mask[n];
+      kind = a.table[id].kind;
+    }
+    if (kind != OLBGFX_NO_DYNAMICS) {
+      A f[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        f[i] = St::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)));
+      }
+
+      if constexpr (MODE >= 0) {
+        counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
+      } else {
+        const DynEntry<A> e = a.table[id];
+        counted = apply_any(f, e, n, a, st_rho, st_usqr);
+      }
+
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
+      }
+
+      if (a.periodic) {
+        const bool lo[3] = {(a.periodic & 1) && x == 0, (a.periodic & 2) && y == 0,
+                            (a.periodic & 4) && z == 0};
+        const bool hi[3] = {(a.periodic & 1) && x == (uint32_t)a.nx - 1,
+                            (a.periodic & 2) && y == (uint32_t)a.ny - 1,
+                            (a.periodic & 4) && z == (uint32_t)a.nz - 1};
+        if (lo[0] | lo[1] | lo[2] | hi[0] | hi[1] | hi[2]) {
+          const int64_t off[3] = {(int64_t)a.nx * a.py * a.pz, (int64_t)a.ny * a.pz,
+                                  (int64_t)a.nz};
+          ghost_all<S>(a.pop, a.stride, a.count, sh, (int64_t)n, lo, hi, off, f,
+                       std::make_integer_sequence<int, Q - 1>{});
+        }
+      }
+    }
+  }
+
+  if constexpr (STATS) {
+    const double c = counted ? 1.0 : 0.0;
+    const double r = counted ? (double)st_rho : 0.0;
+    const double u2 = counted ? (double)st_usqr : 0.0;
+    block_reduce4(c, r, u2, u2, a.stats + 4 * (size_t)lb, in_grid);
+  }
+}
+
+// sums the per-block statistics in a fixed order (deterministic)
+__global__ void __launch_bounds__(256) stats_finalize(const double* partial, uint32_t nblocks,
+                                                      double* out)
+{
+  double c = 0, r = 0, e = 0, m = 0;
+  for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) {
+    c += partial[4 * (size_t)k + 0];
+    r += partial[4 * (size_t)k + 1];
+    e += partial[4 * (size_t)k + 2];
+    m = fmax(m, partial[4 * (size_t)k + 3]);
+  }
+  block_reduce4(c, r, e, m, out, true);
+}
+
+// advance the device offsets by one step: s_i -= d_i (mod count)
+__global__ void advance_shifts(uint32_t* shift, const int64_t* dmod, uint32_t count)
+{
+  const int i = threadIdx.x;
+  if (i < Q) {
+    const uint64_t s = (uint64_t)shift[i] + (uint64_t)dmod[i];
+    shift[i] = (uint32_t)(s >= count ? s - count : s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set-up and evaluation kernels (one thread per padded cell)
+
+template <typename S, typename H>
+__global__ void k_scatter_pop(S* ring, uint32_t count, uint32_t shift, const H* src)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < count) {
+    uint32_t s = n + shift;
+    s = s >= count ? s - count : s;
+    ring[s] = Storage<S>::pack((A)src[n]);
+  }
+}
+
+template <typename S, typename H>
+__global__ void k_gather_pop(const S* ring, uint32_t count, uint32_t shift, H* dst)
+{
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < count) {
+    uint32_t s = n + shift;
+    s = s >= count ? s - count : s;
+    dst[n] = (H)Storage<S>::unpack(ring[s]);
+  }
+}
+
+// iniEquilibrium: f_bar_i = feq_i(rho, u) in the reference's own form
+// rho t_i (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) - t_i (lbm.h:51-64)
+template <typename S, typename H>
+__global__ void k_ini_equilibrium(S* pop, uint64_t stride, uint32_t count, const uint32_t* shift,
+                                  const uint8_t* sel, const H* rho_in, const H* u_in)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= count || (sel && !sel[n])) {
+    return;
+  }
+  const A rho = (A)rho_in[n];
+  const A u[3] = {(A)u_in[3 * (size_t)n], (A)u_in[3 * (size_t)n + 1], (A)u_in[3 * (size_t)n + 2]};
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  constexpr int cx[Q] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+  constexpr int cy[Q] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+  constexpr int cz[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const A t = A(1) / A(TDEN(i));
+    const A cu = A(cx[i]) * u[0] + A(cy[i]) * u[1] + A(cz[i]) * u[2];
+    const A feq = rho * t * (A(1) + A(3) * cu + A(4.5) * cu * cu - A(1.5) * usqr) - t;
+    uint32_t s = n + shift[i];
+    s = s >= count ? s - count : s;
+    pop[(uint64_t)i * stride + s] = Storage<S>::pack(feq);
+  }
+}
+
+// density and velocity per padded cell
+template <typename S, typename H>
+__global__ void k_rho_u(const S* pop, uint64_t stride, uint32_t count, const uint32_t* shift,
+                        H* rho_out, H* u_out)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= count) {
+    return;
+  }
+  A f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    uint32_t s = n + shift[i];
+    s = s >= count ? s - count : s;
+    f[i] = Storage<S>::unpack(pop[(uint64_t)i * stride + s]);
+  }
+  A drho, jx, jy, jz;
+  moments(f, drho, jx, jy, jz);
+  const A rho = drho + A(1);
+  rho_out[n] = (H)rho;
+  u_out[3 * (size_t)n + 0] = (H)(jx / rho);
+  u_out[3 * (size_t)n + 1] = (H)(jy / rho);
+  u_out[3 * (size_t)n + 2] = (H)(jz / rho);
+}
+
+// FP64 moments of every colliding core cell, summed from the stored values
+template <typename S>
+__global__ void __launch_bounds__(256) k_moments(const S* pop, uint64_t stride, uint32_t count,
+                                                 const uint32_t* shift, const uint8_t* mask,
+                                                 const int32_t* kinds, int32_t nx, int32_t ny,
+                                                 int32_t nz, int32_t pad, uint32_t py, uint32_t pz,
+                                                 double* partial)
+{
+  const uint64_t ncore = (uint64_t)nx * ny * nz;
+  const uint64_t lin = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double c = 0, r = 0, jx = 0, jy = 0, jz = 0, u2 = 0, m = 0;
+  if (lin < ncore) {
+    const uint32_t z = (uint32_t)(lin % nz);
+    const uint32_t y = (uint32_t)((lin / nz) % ny);
+    const uint32_t x = (uint32_t)(lin / ((uint64_t)nz * ny));
+    const uint32_t n = ((x + pad) * py + (y + pad)) * pz + (z + pad);
+    if (kinds[mask[n]] != OLBGFX_NO_DYNAMICS) {
+      constexpr int cx[Q] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+      constexpr int cy[Q] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+      constexpr int cz[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        uint32_t s = n + shift[i];
+        s = s >= count ? s - count : s;
+        const double v = (double)Storage<S>::unpack(pop[(uint64_t)i * stride + s]);
+        r += v;
+        jx += cx[i] * v;
+        jy += cy[i] * v;
+        jz += cz[i] * v;
+      }
+      c = 1;
+      const double rho = r + 1.0;
+      u2 = (jx * jx + jy * jy + jz * jz) / (rho * rho);
+      m = u2;
+    }
+  }
+  __shared__ double part[4][8];
+  double vals[7] = {c, r, jx, jy, jz, u2, m};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vals[k] = wave_sum(vals[k]);
+  vals[6] = wave_max(vals[6]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    for (int k = 0; k < 7; ++k) part[wave][k] = vals[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double o[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      for (int k = 0; k < 6; ++k) o[k] += part[w][k];
+      o[6] = fmax(o[6], part[w][6]);
+    }
+    for (int k = 0; k < 7; ++k) partial[8 * (size_t)blockIdx.x + k] = o[k];
+    partial[8 * (size_t)blockIdx.x + 7] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+thread_local std::string g_error;
+
+olbgfx_status fail(olbgfx_status s, const std::string& msg)
+{
+  g_error = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      return fail(OLBGFX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    }                                                                                    \
+  } while (0)
+
+uint64_t magic_for(uint32_t d)
+{
+  if (d <= 1) {
+    return 0;
+  }
+  // ceil(2^64 / d) = floor((2^64 - 1) / d) + 1 for d not a power of two;
+  // for powers of two the same expression is also correct.
+  return UINT64_MAX / d + 1;
+}
+
+}  // namespace
+
+struct olbgfx_lattice {
+  olbgfx_lattice_desc desc{};
+  int elem = 4;               // bytes per stored population
+  uint32_t count = 0;         // padded cells
+  uint64_t stride = 0;        // ring stride in elements
+  uint32_t py = 0, pz = 0;
+  int64_t d[Q] = {};
+  int64_t dmod[Q] = {};       // (-d_i) mod count
+  uint32_t shift[Q] = {};     // host mirror of the offsets
+  void* pop = nullptr;
+  uint8_t* mask = nullptr;
+  uint32_t* shift_dev = nullptr;
+  int64_t* dmod_dev = nullptr;
+  int32_t* kinds_dev = nullptr;
+  void* cell_u = nullptr;
+  void* cell_rho = nullptr;
+  void* cell_force = nullptr;
+  double* stats_partial = nullptr;
+  double* stats_out = nullptr;
+  uint32_t stats_capacity = 0;
+  bool stats_on = false;
+  bool stats_valid = false;
+  int32_t uniform_id = 0;     // all cells start with id 0 = NoDynamics
+  uint32_t block_threads = 256;
+  olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
+  std::vector<uint8_t> host_mask;
+};
+
+namespace {
+
+bool is_f64(const olbgfx_lattice* L) { return L->desc.storage == OLBGFX_STORE_F64; }
+
+template <typename A>
+void fill_table(const olbgfx_lattice* L, DynEntry<A>* t)
+{
+  for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
+    const olbgfx_dynamics_desc& s = L->table[k];
+    t[k].kind = s.kind;
+    t[k].omega = (A)s.omega;
+    // collision.h:408: omega2 = 1 / (magic / (1/omega - 0.5) + 0.5), formed in
+    // double from the value-type omega and magic
+    const double om = (double)(A)s.omega;
+    const double mg = (double)(A)s.magic;
+    const double om2 = (om != 0.0) ? 1.0 / (mg / ((double)((A)1 / (A)om) - 0.5) + 0.5) : 0.0;
+    t[k].omega2 = (A)om2;
+    for (int ax = 0; ax < 3; ++ax) {
+      t[k].force[ax] = (A)s.force[ax];
+      t[k].wall_u[ax] = (A)s.wall_velocity[ax];
+    }
+    t[k].wall_rho = (A)s.wall_rho;
+  }
+}
+
+template <typename A>
+olbgfx_status fill_args(olbgfx_lattice* L, StepArgs<A>& a, bool device_shifts)
+{
+  const olbgfx_lattice_desc& D = L->desc;
+  a.pop = L->pop;
+  a.stride = L->stride;
+  a.count = L->count;
+  for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
+  a.shift_dev = device_shifts ? L->shift_dev : nullptr;
+  a.mask = L->mask;
+  a.uniform_id = L->uniform_id;
+  a.nx = D.nx;
+  a.ny = D.ny;
+  a.nz = D.nz;
+  a.pad = D.padding;
+  a.py = L->py;
+  a.pz = L->pz;
+  a.nyz = (uint32_t)D.ny * (uint32_t)D.nz;
+  a.magic_nyz = magic_for(a.nyz);
+  a.magic_nz = magic_for((uint32_t)D.nz);
+  a.ncore = (uint32_t)D.nx * a.nyz;
+  a.nblocks = (a.ncore + L->block_threads - 1) / L->block_threads;
+  a.blocks_per_xcd = (a.nblocks + 7) / 8;
+  a.periodic = (D.periodic[0] ? 1 : 0) | (D.periodic[1] ? 2 : 0) | (D.periodic[2] ? 4 : 0);
+  a.cell_u = (const A*)L->cell_u;
+  a.cell_rho = (const A*)L->cell_rho;
+  a.cell_force = (const A*)L->cell_force;
+  a.stats = L->stats_partial;
+  fill_table(L, a.table);
+  return OLBGFX_OK;
+}
+
+template <typename S, bool STATS>
+void launch_mode(int mode, dim3 grid, dim3 block, hipStream_t stream,
+                 const StepArgs<typename Storage<S>::A>& a)
+{
+  switch (mode) {
+  case OLBGFX_NO_DYNAMICS:
+    break;
+  case OLBGFX_BGK:
+    step_kernel<S, OLBGFX_BGK, STATS><<<grid, block, 0, stream>>>(a);
+    break;
+  case OLBGFX_TRT:
+    step_kernel<S, OLBGFX_TRT, STATS><<<grid, block, 0, stream>>>(a);
+    break;
+  case OLBGFX_FORCED_BGK:
+    step_kernel<S, OLBGFX_FORCED_BGK, STATS><<<grid, block, 0, stream>>>(a);
+    break;
+  case OLBGFX_FORCED_TRT:
+    step_kernel<S, OLBGFX_FORCED_TRT, STATS><<<grid, block, 0, stream>>>(a);
+    break;
+  default:
+    step_kernel<S, -1, STATS><<<grid, block, 0, stream>>>(a);
+    break;
+  }
+}
+
+template <typename S>
+olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream)
+{
+  using A = typename Storage<S>::A;
+  StepArgs<A> a;
+  fill_args(L, a, false);
+  if (a.ncore == 0) {
+    return OLBGFX_OK;
+  }
+  if (L->stats_on && a.nblocks > L->stats_capacity) {
+    if (L->stats_partial) {
+      HIP_TRY(hipFree(L->stats_partial));
+      L->stats_partial = nullptr;
+    }
+    HIP_TRY(hipMalloc(&L->stats_partial, 4 * sizeof(double) * (size_t)a.nblocks));
+    L->stats_capacity = a.nblocks;
+    a.stats = L->stats_partial;
+  }
+  const dim3 grid(8 * a.blocks_per_xcd);
+  const dim3 block(L->block_threads);
+  const int mode = L->uniform_id >= 0 ? L->table[L->uniform_id].kind : -1;
+  for (uint32_t k = 0; k < steps; ++k) {
+    for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
+    if (L->stats_on) {
+      launch_mode<S, true>(mode, grid, block, stream, a);
+      stats_finalize<<<1, 256, 0, stream>>>(L->stats_partial, a.nblocks, L->stats_out);
+    } else {
+      launch_mode<S, false>(mode, grid, block, stream, a);
+    }
+    HIP_TRY(hipGetLastError());
+    // stream(): s_i -= d_i (mod count), core/blockLattice.hh:316-338
+    for (int i = 0; i < Q; ++i) {
+      const uint64_t s = (uint64_t)L->shift[i] + (uint64_t)L->dmod[i];
+      L->shift[i] = (uint32_t)(s >= L->count ? s - L->count : s);
+    }
+  }
+  if (steps > 0) {
+    L->stats_valid = L->stats_on;
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status sync_shifts(olbgfx_lattice* L)
+{
+  HIP_TRY(hipMemcpy(L->shift_dev, L->shift, sizeof(L->shift), hipMemcpyHostToDevice));
+  return OLBGFX_OK;
+}
+
+template <typename S, typename H>
+olbgfx_status transfer_pop(olbgfx_lattice* L, void* host, bool to_device)
+{
+  H* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, sizeof(H) * (size_t)L->count));
+  const dim3 block(256);
+  const dim3 grid((L->count + 255) / 256);
+  olbgfx_status st = OLBGFX_OK;
+  for (int i = 0; i < Q && st == OLBGFX_OK; ++i) {
+    S* ring = (S*)L->pop + (uint64_t)i * L->stride;
+    H* h = (H*)host + (size_t)i * L->count;
+    if (to_device) {
+      if (hipMemcpy(tmp, h, sizeof(H) * (size_t)L->count, hipMemcpyHostToDevice) != hipSuccess) {
+        st = fail(OLBGFX_ERR_HIP, "population upload failed");
+        break;
+      }
+      k_scatter_pop<S, H><<<grid, block>>>(ring, L->count, L->shift[i], tmp);
+    } else {
+      k_gather_pop<S, H><<<grid, block>>>(ring, L->count, L->shift[i], tmp);
+      if (hipMemcpy(h, tmp, sizeof(H) * (size_t)L->count, hipMemcpyDeviceToHost) != hipSuccess) {
+        st = fail(OLBGFX_ERR_HIP, "population download failed");
+        break;
+      }
+    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess && st == OLBGFX_OK) {
+    st = fail(OLBGFX_ERR_HIP, "population transfer kernel failed");
+  }
+  hipFree(tmp);
+  return st;
+}
+
+template <typename S>
+olbgfx_status transfer_pop_any(olbgfx_lattice* L, int32_t vt, void* host, bool to_device)
+{
+  if (vt == OLBGFX_VALUE_F32) {
+    return transfer_pop<S, float>(L, host, to_device);
+  }
+  if (vt == OLBGFX_VALUE_F64) {
+    return transfer_pop<S, double>(L, host, to_device);
+  }
+  return fail(OLBGFX_ERR_INVALID, "unknown value type");
+}
+
+template <typename F>
+olbgfx_status dispatch_storage(olbgfx_lattice* L, F&& f)
+{
+  switch (L->desc.storage) {
+  case OLBGFX_STORE_F32:
+    return f((float*)nullptr);
+  case OLBGFX_STORE_F16:
+    return f((_Float16*)nullptr);
+  case OLBGFX_STORE_BF16:
+    return f((bf16_t*)nullptr);
+  case OLBGFX_STORE_F64:
+    return f((double*)nullptr);
+  default:
+    return fail(OLBGFX_ERR_INVALID, "unknown storage type");
+  }
+}
+
+size_t value_size(int32_t vt) { return vt == OLBGFX_VALUE_F64 ? 8 : 4; }
+
+// copies a host array of n values (F32 or F64) into device memory of the
+// lattice's arithmetic type
+olbgfx_status upload_values(olbgfx_lattice* L, void** dst, int32_t vt, const void* src, size_t n)
+{
+  if (vt != OLBGFX_VALUE_F32 && vt != OLBGFX_VALUE_F64) {
+    return fail(OLBGFX_ERR_INVALID, "unknown value type");
+  }
+  const bool f64 = is_f64(L);
+  const size_t esz = f64 ? 8 : 4;
+  std::vector<unsigned char> conv(n * esz);
+  for (size_t k = 0; k < n; ++k) {
+    const double v = vt == OLBGFX_VALUE_F64 ? ((const double*)src)[k] : (double)((const float*)src)[k];
+    if (f64) {
+      ((double*)conv.data())[k] = v;
+    } else {
+      ((float*)conv.data())[k] = vt == OLBGFX_VALUE_F32 ? ((const float*)src)[k] : (float)v;
+    }
+  }
+  if (!*dst) {
+    HIP_TRY(hipMalloc(dst, n * esz));
+  }
+  HIP_TRY(hipMemcpy(*dst, conv.data(), n * esz, hipMemcpyHostToDevice));
+  return OLBGFX_OK;
+}
+
+olbgfx_status check(const olbgfx_lattice* L)
+{
+  if (!L) {
+    return fail(OLBGFX_ERR_INVALID, "null lattice handle");
+  }
+  return OLBGFX_OK;
+}
+
+void recompute_uniform(olbgfx_lattice* L)
+{
+  const olbgfx_lattice_desc& D = L->desc;
+  int id = -2;
+  for (int x = 0; x < D.nx && id != -1; ++x) {
+    for (int y = 0; y < D.ny && id != -1; ++y) {
+      const size_t base = ((size_t)(x + D.padding) * L->py + (y + D.padding)) * L->pz + D.padding;
+      for (int z = 0; z < D.nz; ++z) {
+        const int v = L->host_mask[base + z];
+        if (id == -2) {
+          id = v;
+        } else if (v != id) {
+          id = -1;
+          break;
+        }
+      }
+    }
+  }
+  L->uniform_id = id < 0 ? -1 : id;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* olbgfx_last_error(void) { return g_error.c_str(); }
+
+olbgfx_status olbgfx_lattice_create(const olbgfx_lattice_desc* desc, olbgfx_lattice** out)
+{
+  if (!desc || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  *out = nullptr;
+  const olbgfx_lattice_desc& D = *desc;
+  if (D.nx <= 0 || D.ny <= 0 || D.nz <= 0) {
+    return fail(OLBGFX_ERR_INVALID, "lattice extent must be positive");
+  }
+  if (D.padding < 1) {
+    return fail(OLBGFX_ERR_INVALID, "padding must be at least 1 for D3Q19");
+  }
+  if (D.storage < OLBGFX_STORE_F32 || D.storage > OLBGFX_STORE_F64) {
+    return fail(OLBGFX_ERR_INVALID, "unknown storage type");
+  }
+  const uint64_t px = (uint64_t)D.nx + 2 * (uint64_t)D.padding;
+  const uint64_t py = (uint64_t)D.ny + 2 * (uint64_t)D.padding;
+  const uint64_t pz = (uint64_t)D.nz + 2 * (uint64_t)D.padding;
+  const uint64_t count = px * py * pz;
+  // the reference limits CellID to uint32 (core/blockStructure.h:53-55); the
+  // kernels add two cell indices in 32 bits, so stay below 2^31
+  if (count >= (1ull << 31)) {
+    return fail(OLBGFX_ERR_INVALID, "cell count must stay below 2^31");
+  }
+  // ring slots are addressed with 32-bit byte offsets
+  const uint64_t elem = D.storage == OLBGFX_STORE_F32 ? 4 : (D.storage == OLBGFX_STORE_F64 ? 8 : 2);
+  if (count * elem >= (1ull << 32)) {
+    return fail(OLBGFX_ERR_INVALID, "one population ring must stay below 4 GiB");
+  }
+  olbgfx_lattice* L = new (std::nothrow) olbgfx_lattice();
+  if (!L) {
+    return fail(OLBGFX_ERR_NOMEM, "host allocation failed");
+  }
+  L->desc = D;
+  L->elem = D.storage == OLBGFX_STORE_F32 ? 4 : (D.storage == OLBGFX_STORE_F64 ? 8 : 2);
+  L->count = (uint32_t)count;
+  L->py = (uint32_t)py;
+  L->pz = (uint32_t)pz;
+  const uint64_t align = 256 / L->elem;
+  L->stride = (count + align - 1) / align * align;
+  const int64_t proj[3] = {(int64_t)(py * pz), (int64_t)pz, 1};
+  for (int i = 0; i < Q; ++i) {
+    L->d[i] = CX(i) * proj[0] + CY(i) * proj[1] + CZ(i) * proj[2];
+    int64_t m = (-L->d[i]) % (int64_t)count;
+    if (m < 0) m += (int64_t)count;
+    L->dmod[i] = m;
+    L->shift[i] = 0;
+  }
+  for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
+    L->table[k] = olbgfx_dynamics_desc{};
+    L->table[k].kind = OLBGFX_NO_DYNAMICS;
+    L->table[k].wall_rho = 1.0;
+  }
+  L->host_mask.assign(count, 0);
+  L->uniform_id = 0;
+
+  auto cleanup = [&](olbgfx_status s) {
+    olbgfx_lattice_destroy(L);
+    return s;
+  };
+  const size_t bytes = (size_t)Q * L->stride * L->elem;
+  if (hipMalloc(&L->pop, bytes) != hipSuccess) {
+    return cleanup(fail(OLBGFX_ERR_NOMEM, "population allocation failed"));
+  }
+  if (hipMemset(L->pop, 0, bytes) != hipSuccess ||
+      hipMalloc(&L->mask, count) != hipSuccess || hipMemset(L->mask, 0, count) != hipSuccess ||
+      hipMalloc(&L->shift_dev, sizeof(L->shift)) != hipSuccess ||
+      hipMemset(L->shift_dev, 0, sizeof(L->shift)) != hipSuccess ||
+      hipMalloc(&L->dmod_dev, sizeof(L->dmod)) != hipSuccess ||
+      hipMemcpy(L->dmod_dev, L->dmod, sizeof(L->dmod), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&L->kinds_dev, sizeof(int32_t) * 256) != hipSuccess ||
+      hipMemset(L->kinds_dev, 0, sizeof(int32_t) * 256) != hipSuccess ||
+      hipMalloc(&L->stats_out, 4 * sizeof(double)) != hipSuccess ||
+      hipMemset(L->stats_out, 0, 4 * sizeof(double)) != hipSuccess) {
+    return cleanup(fail(OLBGFX_ERR_NOMEM, "device allocation failed"));
+  }
+  *out = L;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
+{
+  if (!L) {
+    return OLBGFX_OK;
+  }
+  hipDeviceSynchronize();
+  hipFree(L->pop);
+  hipFree(L->mask);
+  hipFree(L->shift_dev);
+  hipFree(L->dmod_dev);
+  hipFree(L->kinds_dev);
+  hipFree(L->cell_u);
+  hipFree(L->cell_rho);
+  hipFree(L->cell_force);
+  hipFree(L->stats_partial);
+  hipFree(L->stats_out);
+  delete L;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_lattice_ncells(const olbgfx_lattice* L, uint64_t* n)
+{
+  if (check(L) != OLBGFX_OK || !n) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  *n = L->count;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_neighbor_distances(const olbgfx_lattice* L, int64_t out[19])
+{
+  if (check(L) != OLBGFX_OK || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  for (int i = 0; i < Q; ++i) out[i] = L->d[i];
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ring_shifts(olbgfx_lattice* L, int64_t out[19])
+{
+  if (check(L) != OLBGFX_OK || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  for (int i = 0; i < Q; ++i) out[i] = L->shift[i];
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_dynamics_desc* dyn)
+{
+  if (check(L) != OLBGFX_OK || !dyn) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  if (id >= OLBGFX_MAX_DYNAMICS) {
+    return fail(OLBGFX_ERR_INVALID, "dynamics id out of range");
+  }
+  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_BOUNCE_BACK_VELOCITY) {
+    return fail(OLBGFX_ERR_UNSUPPORTED, "dynamics kind not provided by the gfx950 platform");
+  }
+  L->table[id] = *dyn;
+  int32_t kinds[256];
+  for (int k = 0; k < 256; ++k) kinds[k] = k < OLBGFX_MAX_DYNAMICS ? L->table[k].kind : 0;
+  HIP_TRY(hipMemcpy(L->kinds_dev, kinds, sizeof(kinds), hipMemcpyHostToDevice));
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* L, const uint8_t* ids)
+{
+  if (check(L) != OLBGFX_OK || !ids) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  for (uint32_t n = 0; n < L->count; ++n) {
+    if (ids[n] >= OLBGFX_MAX_DYNAMICS) {
+      return fail(OLBGFX_ERR_INVALID, "dynamics id out of range in mask");
+    }
+  }
+  std::memcpy(L->host_mask.data(), ids, L->count);
+  HIP_TRY(hipMemcpy(L->mask, ids, L->count, hipMemcpyHostToDevice));
+  recompute_uniform(L);
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* L, int32_t field, int32_t vt, const void* values)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  void** dst = nullptr;
+  size_t comps = 0;
+  switch (field) {
+  case OLBGFX_FIELD_VELOCITY:
+    dst = &L->cell_u;
+    comps = 3;
+    break;
+  case OLBGFX_FIELD_RHO:
+    dst = &L->cell_rho;
+    comps = 1;
+    break;
+  case OLBGFX_FIELD_FORCE:
+    dst = &L->cell_force;
+    comps = 3;
+    break;
+  default:
+    return fail(OLBGFX_ERR_INVALID, "unknown field");
+  }
+  if (!values) {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(*dst));
+    *dst = nullptr;
+    return OLBGFX_OK;
+  }
+  return upload_values(L, dst, vt, values, comps * (size_t)L->count);
+}
+
+olbgfx_status olbgfx_ini_equilibrium(olbgfx_lattice* L, const uint8_t* sel, int32_t vt,
+                                     const void* rho, const void* u)
+{
+  if (check(L) != OLBGFX_OK || !rho || !u) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  if (vt != OLBGFX_VALUE_F32 && vt != OLBGFX_VALUE_F64) {
+    return fail(OLBGFX_ERR_INVALID, "unknown value type");
+  }
+  const size_t n = L->count;
+  const size_t vs = value_size(vt);
+  void* d_rho = nullptr;
+  void* d_u = nullptr;
+  uint8_t* d_sel = nullptr;
+  olbgfx_status st = OLBGFX_OK;
+  if (hipMalloc(&d_rho, n * vs) != hipSuccess || hipMalloc(&d_u, 3 * n * vs) != hipSuccess ||
+      (sel && hipMalloc(&d_sel, n) != hipSuccess)) {
+    st = fail(OLBGFX_ERR_NOMEM, "temporary allocation failed");
+  }
+  if (st == OLBGFX_OK &&
+      (hipMemcpy(d_rho, rho, n * vs, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(d_u, u, 3 * n * vs, hipMemcpyHostToDevice) != hipSuccess ||
+       (sel && hipMemcpy(d_sel, sel, n, hipMemcpyHostToDevice) != hipSuccess) ||
+       sync_shifts(L) != OLBGFX_OK)) {
+    st = fail(OLBGFX_ERR_HIP, "upload failed");
+  }
+  if (st == OLBGFX_OK) {
+    const dim3 block(256), grid((L->count + 255) / 256);
+    st = dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      if (vt == OLBGFX_VALUE_F32) {
+        k_ini_equilibrium<S, float><<<grid, block>>>((S*)L->pop, L->stride, L->count, L->shift_dev,
+                                                     d_sel, (const float*)d_rho, (const float*)d_u);
+      } else {
+        k_ini_equilibrium<S, double><<<grid, block>>>((S*)L->pop, L->stride, L->count,
+                                                      L->shift_dev, d_sel, (const double*)d_rho,
+                                                      (const double*)d_u);
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipDeviceSynchronize());
+      return OLBGFX_OK;
+    });
+  }
+  hipFree(d_rho);
+  hipFree(d_u);
+  hipFree(d_sel);
+  return st;
+}
+
+olbgfx_status olbgfx_get_populations(olbgfx_lattice* L, int32_t vt, void* host)
+{
+  if (check(L) != OLBGFX_OK || !host) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return dispatch_storage(L, [&](auto* tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    return transfer_pop_any<S>(L, vt, host, false);
+  });
+}
+
+olbgfx_status olbgfx_set_populations(olbgfx_lattice* L, int32_t vt, const void* host)
+{
+  if (check(L) != OLBGFX_OK || !host) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return dispatch_storage(L, [&](auto* tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    return transfer_pop_any<S>(L, vt, const_cast<void*>(host), true);
+  });
+}
+
+olbgfx_status olbgfx_collide_and_stream(olbgfx_lattice* L, uint32_t steps, void* stream)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_storage(L, [&](auto* tag) {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    return launch_steps<S>(L, steps, s);
+  });
+}
+
+olbgfx_status olbgfx_set_statistics(olbgfx_lattice* L, int32_t on)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  L->stats_on = on != 0;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_get_statistics(olbgfx_lattice* L, double out[4])
+{
+  if (check(L) != OLBGFX_OK || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  if (!L->stats_valid) {
+    for (int k = 0; k < 4; ++k) out[k] = 0;
+    return OLBGFX_OK;
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(out, L->stats_out, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* L, double out[8], void* stream)
+{
+  if (check(L) != OLBGFX_OK || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  const olbgfx_lattice_desc& D = L->desc;
+  const uint64_t ncore = (uint64_t)D.nx * D.ny * D.nz;
+  const uint32_t nb = (uint32_t)((ncore + 255) / 256);
+  double* partial = nullptr;
+  HIP_TRY(hipMalloc(&partial, 8 * sizeof(double) * (size_t)nb));
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  if (sync_shifts(L) != OLBGFX_OK) {
+    hipFree(partial);
+    return OLBGFX_ERR_HIP;
+  }
+  olbgfx_status st = dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    k_moments<S><<<nb, 256, 0, s>>>((const S*)L->pop, L->stride, L->count, L->shift_dev, L->mask,
+                                    L->kinds_dev, D.nx, D.ny, D.nz, D.padding, L->py, L->pz,
+                                    partial);
+    HIP_TRY(hipGetLastError());
+    return OLBGFX_OK;
+  });
+  if (st == OLBGFX_OK) {
+    std::vector<double> h(8 * (size_t)nb);
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(h.data(), partial, h.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+      st = fail(OLBGFX_ERR_HIP, "moment reduction failed");
+    } else {
+      double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t b = 0; b < nb; ++b) {
+        for (int k = 0; k < 6; ++k) o[k] += h[8 * (size_t)b + k];
+        o[6] = std::max(o[6], h[8 * (size_t)b + 6]);
+      }
+      for (int k = 0; k < 8; ++k) out[k] = o[k];
+    }
+  }
+  hipFree(partial);
+  return st;
+}
+
+olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* L, int32_t vt, void* rho, void* u)
+{
+  if (check(L) != OLBGFX_OK || !rho || !u) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  if (vt != OLBGFX_VALUE_F32 && vt != OLBGFX_VALUE_F64) {
+    return fail(OLBGFX_ERR_INVALID, "unknown value type");
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  const size_t n = L->count;
+  const size_t vs = value_size(vt);
+  void* d_rho = nullptr;
+  void* d_u = nullptr;
+  HIP_TRY(hipMalloc(&d_rho, n * vs));
+  if (hipMalloc(&d_u, 3 * n * vs) != hipSuccess) {
+    hipFree(d_rho);
+    return fail(OLBGFX_ERR_NOMEM, "temporary allocation failed");
+  }
+  olbgfx_status st = sync_shifts(L);
+  if (st == OLBGFX_OK) {
+    const dim3 block(256), grid((L->count + 255) / 256);
+    st = dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
+      using S = std::remove_pointer_t<decltype(tag)>;
+      if (vt == OLBGFX_VALUE_F32) {
+        k_rho_u<S, float><<<grid, block>>>((const S*)L->pop, L->stride, L->count, L->shift_dev,
+                                           (float*)d_rho, (float*)d_u);
+      } else {
+        k_rho_u<S, double><<<grid, block>>>((const S*)L->pop, L->stride, L->count, L->shift_dev,
+                                            (double*)d_rho, (double*)d_u);
+      }
+      HIP_TRY(hipGetLastError());
+      return OLBGFX_OK;
+    });
+  }
+  if (st == OLBGFX_OK &&
+      (hipMemcpy(rho, d_rho, n * vs, hipMemcpyDeviceToHost) != hipSuccess ||
+       hipMemcpy(u, d_u, 3 * n * vs, hipMemcpyDeviceToHost) != hipSuccess)) {
+    st = fail(OLBGFX_ERR_HIP, "download failed");
+  }
+  hipFree(d_rho);
+  hipFree(d_u);
+  return st;
+}
+
+olbgfx_status olbgfx_set_launch(olbgfx_lattice* L, uint32_t block_threads)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (block_threads == 0) {
+    block_threads = 256;
+  }
+  // the step kernel is compiled with __launch_bounds__(kMaxBlock)
+  if (block_threads < 64 || block_threads > kMaxBlock || block_threads % 64 != 0) {
+    return fail(OLBGFX_ERR_INVALID, "block size must be a multiple of 64 in [64, 256]");
+  }
+  L->block_threads = block_threads;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_synchronize(olbgfx_lattice* L)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return OLBGFX_OK;
+}
+
+}  // extern "C"

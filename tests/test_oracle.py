@@ -1,0 +1,247 @@
+"""Pins the CPU oracle (oracle/olb_oracle.c) by closed forms, invariants and
+special cases, so a dropped term, wrong sign or index fails here.  The
+reference's own fixtures (tests/golden/) are checked in test_golden.py."""
+import numpy as np
+import pytest
+
+from openlb_amd import (BGK, BOUNCE_BACK, FORCED_BGK, FORCED_TRT, NO_DYNAMICS, TRT, Dynamics,
+                        cases)
+from oracle.oracle import OracleLattice, f32_to_bf16_bits, f32_to_f16_bits
+from tests.helpers import C, OPP, T, velocity
+
+
+def test_cell_id_and_neighbor_distance_closed_form():
+    """CellID = ((x+p)Py + (y+p))Pz + (z+p), d_i = c_i . (PyPz, Pz, 1)
+    (core/blockStructure.h:150-182); bit-exact integers."""
+    L = OracleLattice((5, 7, 3), padding=2)
+    Px, Py, Pz = 9, 11, 7
+    assert L.ncells == Px * Py * Pz
+    for x, y, z in [(0, 0, 0), (4, 6, 2), (-2, -2, -2), (6, 8, 4), (1, 3, 2)]:
+        expect = ((x + 2) * Py + (y + 2)) * Pz + (z + 2)
+        assert L.cell_id(x, y, z) == expect
+        assert expect == np.ravel_multi_index((x + 2, y + 2, z + 2), (Px, Py, Pz))
+    d = L.neighbor_distances()
+    np.testing.assert_array_equal(d, C @ np.array([Py * Pz, Pz, 1]))
+    assert d[0] == 0 and np.all(d[1:10] == -d[10:19])
+
+
+def test_d3q19_lattice_constants():
+    """weights sum to 1, isotropy sum t c c = I/3, opposite reverses c"""
+    assert abs(T.sum() - 1.0) < 1e-15
+    np.testing.assert_allclose((T[:, None, None] * C[:, :, None] * C[:, None, :]).sum(0),
+                               np.eye(3) / 3, atol=1e-15)
+    np.testing.assert_array_equal(C[OPP], -C)
+
+
+def test_periodic_shift_streaming_is_a_pull():
+    """With NoDynamics everywhere a step only rotates the rings:
+    f_i(n) after = f_i(n - d_i) before (mod count), and the ring offsets follow
+    s_i(t) = -t d_i mod count (core/platform/cpu/sisd/column.h:193-204)."""
+    L = OracleLattice((4, 5, 6), padding=1, storage="f64")
+    rng = np.random.default_rng(1)
+    f0 = rng.standard_normal((19, L.ncells))
+    L.set_populations(f0)
+    L.collide_and_stream(1)
+    f1 = L.get_populations()
+    d = L.neighbor_distances()
+    n = np.arange(L.ncells)
+    for i in range(19):
+        np.testing.assert_array_equal(f1[i], f0[i][(n - d[i]) % L.ncells])
+    L.collide_and_stream(4)
+    np.testing.assert_array_equal(L.ring_shifts(), (-5 * d) % L.ncells)
+
+
+@pytest.mark.parametrize("kind", [BGK, TRT])
+def test_rest_equilibrium_is_a_fixed_point(kind):
+    """rho = 1, u = 0 means f_bar = 0, which every collision keeps exactly."""
+    c = cases.tgv(6)
+    c.dynamics = {1: Dynamics(kind, omega=1.3, magic=0.25)}
+    c.rho[...] = 1.0
+    c.u[...] = 0.0
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(L)
+    L.collide_and_stream(5)
+    assert np.all(L.get_populations() == 0.0)
+
+
+def test_equilibrium_moments():
+    """feq carries the prescribed moments: sum f_bar = rho - 1,
+    sum c f_bar = rho u, sum c c f_bar = rho/3 I + rho u u - I/3."""
+    L = OracleLattice((3, 3, 3), padding=1, storage="f64")
+    rng = np.random.default_rng(7)
+    rho = 1.0 + 0.05 * rng.standard_normal(L.ncells)
+    u = 0.05 * rng.standard_normal((L.ncells, 3))
+    L.ini_equilibrium(rho, u)
+    f = L.get_populations()
+    np.testing.assert_allclose(f.sum(0), rho - 1.0, atol=1e-15)
+    np.testing.assert_allclose((C.T @ f).T, rho[:, None] * u, atol=1e-15)
+    pi = np.einsum("ia,ib,in->nab", C, C, f)
+    expect = (rho[:, None, None] / 3 - 1 / 3) * np.eye(3) + rho[:, None, None] * u[:, :, None] * u[:, None, :]
+    np.testing.assert_allclose(pi, expect, atol=1e-14)
+
+
+def test_bgk_conserves_mass_and_momentum():
+    c = cases.tgv(12)
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+    c.apply(L)
+    core = c.core_mask().reshape(-1)
+    f0 = L.get_populations()[:, core]
+    L.collide_and_stream(40)
+    f1 = L.get_populations()[:, core]
+    assert abs(f1.sum() - f0.sum()) < 1e-11
+    np.testing.assert_allclose(C.T @ f1.sum(1), C.T @ f0.sum(1), atol=1e-12)
+
+
+def test_trt_reduces_to_bgk():
+    """TRT with magic = (1/omega - 1/2)^2 has omega_minus = omega
+    (collision.h:408), i.e. it is BGK."""
+    omega = 1.0 / 0.62
+    magic = (1.0 / omega - 0.5) ** 2
+    out = []
+    for dyn in (Dynamics(BGK, omega=omega), Dynamics(TRT, omega=omega, magic=magic)):
+        c = cases.tgv(10)
+        c.dynamics = {1: dyn}
+        L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+        c.apply(L)
+        L.collide_and_stream(30)
+        out.append(L.get_populations()[:, c.core_mask().reshape(-1)])
+    np.testing.assert_allclose(out[1], out[0], atol=1e-14)
+
+
+def test_trt_odd_relaxation_differs_from_bgk():
+    """with a different magic parameter TRT departs from BGK (the odd part
+    relaxes with omega_minus), while mass and momentum stay conserved"""
+    c = cases.tgv(10)
+    omega = 1.0 / 0.7
+    c.dynamics = {1: Dynamics(TRT, omega=omega, magic=3.0 / 16.0)}
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+    c.apply(L)
+    core = c.core_mask().reshape(-1)
+    f0 = L.get_populations()[:, core]
+    L.collide_and_stream(20)
+    f1 = L.get_populations()[:, core]
+    c2 = cases.tgv(10)
+    c2.dynamics = {1: Dynamics(BGK, omega=omega)}
+    B = OracleLattice(c2.extent, c2.padding, c2.periodic, "f64")
+    c2.apply(B)
+    B.collide_and_stream(20)
+    assert np.max(np.abs(B.get_populations()[:, core] - f1)) > 1e-7
+    assert abs(f1.sum() - f0.sum()) < 1e-11
+    np.testing.assert_allclose(C.T @ f1.sum(1), C.T @ f0.sum(1), atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [FORCED_BGK, FORCED_TRT])
+def test_guo_forcing_adds_rho_f_per_step(kind):
+    """Guo forcing (forcing.h:36-151, lbm.h:567-588): each collision leaves
+    the mass unchanged and adds rho*F*(1 + (omega_odd - omega)/2) to the
+    momentum, where omega_odd relaxes the odd moments (omega for BGK, omega_minus
+    for TRT: the reference's force term uses omega in both cases)."""
+    F = np.array([2e-5, -1e-5, 3e-6])
+    omega, magic = 1.2, 3.0 / 16.0
+    omega_odd = omega if kind == FORCED_BGK else 1.0 / (magic / (1.0 / omega - 0.5) + 0.5)
+    c = cases.tgv(8)
+    c.dynamics = {1: Dynamics(kind, omega=omega, magic=magic, force=tuple(F))}
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+    c.apply(L)
+    core = c.core_mask().reshape(-1)
+    f0 = L.get_populations()[:, core]
+    rho_sum = (f0.sum(0) + 1.0).sum()
+    L.collide_and_stream(1)
+    f1 = L.get_populations()[:, core]
+    assert abs(f1.sum() - f0.sum()) < 1e-13
+    gain = rho_sum * F * (1.0 + (omega_odd - omega) / 2.0)
+    np.testing.assert_allclose(C.T @ f1.sum(1) - C.T @ f0.sum(1), gain, rtol=1e-9, atol=1e-15)
+
+
+def test_bounce_back_is_an_involution():
+    """On a 1^3 periodic block streaming is the identity, so a BounceBack
+    cell swaps f_i and f_opp(i) each step (collision.h:465-484)."""
+    L = OracleLattice((1, 1, 1), padding=1, periodic=(True, True, True), storage="f64")
+    L.set_dynamics(1, Dynamics(BOUNCE_BACK))
+    mask = np.zeros(L.ncells, dtype=np.uint8)
+    n = L.cell_id(0, 0, 0)
+    mask[n] = 1
+    L.set_dynamics_mask(mask)
+    rng = np.random.default_rng(3)
+    f0 = rng.standard_normal((19, L.ncells))
+    L.set_populations(f0)
+    L.collide_and_stream(1)
+    np.testing.assert_array_equal(L.get_populations()[:, n], f0[OPP, n])
+    L.collide_and_stream(1)
+    np.testing.assert_array_equal(L.get_populations()[:, n], f0[:, n])
+
+
+def test_no_dynamics_cells_are_untouched():
+    L = OracleLattice((1, 1, 1), padding=1, periodic=(True, True, True), storage="f64")
+    L.set_dynamics(1, Dynamics(NO_DYNAMICS))
+    rng = np.random.default_rng(4)
+    f0 = rng.standard_normal((19, L.ncells))
+    L.set_populations(f0)
+    L.collide_and_stream(1)
+    n = L.cell_id(0, 0, 0)
+    np.testing.assert_array_equal(L.get_populations()[:, n], f0[:, n])
+
+
+def test_tgv_energy_decay_matches_viscosity():
+    """Low-Re Taylor-Green decay: kinetic energy ~ exp(-2 nu |k|^2 t) with
+    |k|^2 = 3 (2 pi / N)^2 and nu = (tau - 1/2)/3 (core/unitConverter.h:882).
+    The decay rate agrees to within the O(k^2) lattice dispersion (< 1% at
+    N = 32); a wrong viscosity relation would miss by far more."""
+    n, tau, steps = 32, 0.8, 200
+    c = cases.tgv(n, u0=0.005, tau=tau)
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+    c.apply(L)
+    core = c.core_mask().reshape(-1)
+    e0 = np.sum(velocity(L.get_populations()[:, core]) ** 2)
+    L.collide_and_stream(steps)
+    e1 = np.sum(velocity(L.get_populations()[:, core]) ** 2)
+    nu = (tau - 0.5) / 3.0
+    rate = -np.log(e1 / e0) / steps
+    expect = 2.0 * nu * 3.0 * (2 * np.pi / n) ** 2
+    assert abs(rate - expect) / expect < 0.015
+
+
+def test_statistics_match_direct_moments():
+    """LatticeStatistics aggregate (latticeStatistics.h:50-70) equals the
+    moments of the pre-collision populations of the step."""
+    c = cases.tgv(8)
+    L = OracleLattice(c.extent, c.padding, c.periodic, "f64")
+    c.apply(L)
+    core = c.core_mask().reshape(-1)
+    f = L.get_populations()[:, core]
+    L.set_statistics(True)
+    L.collide_and_stream(1)
+    s = L.get_statistics()
+    u = velocity(f)
+    assert s["nCells"] == 512
+    assert abs(s["sumRho"] - (f.sum(0) + 1).sum()) < 1e-10
+    assert abs(s["sumUSqr"] - np.sum(u ** 2)) < 1e-14
+    assert abs(s["maxUSqr"] - np.max(np.sum(u ** 2, 1))) < 1e-16
+
+
+def test_fp16_rounding_matches_ieee_binary16():
+    """the storage emulation rounds like IEEE binary16 round-to-nearest-even
+    (numpy float16), subnormals, overflow and ties included"""
+    rng = np.random.default_rng(11)
+    vals = np.concatenate([
+        rng.standard_normal(2000).astype(np.float32) * 0.05,
+        (rng.standard_normal(500) * 1e-5).astype(np.float32),
+        (rng.standard_normal(200) * 1e-7).astype(np.float32),
+        np.array([65504, 65519.99, 65520, 1e6, -1e6, 0.0, -0.0, 2.0 ** -24, 2.0 ** -25,
+                  3 * 2.0 ** -26, 1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11], dtype=np.float32),
+    ])
+    got = f32_to_f16_bits(vals)
+    expect = vals.astype(np.float16).view(np.uint16)
+    np.testing.assert_array_equal(got, expect)
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(12)
+    vals = np.concatenate([
+        rng.standard_normal(3000).astype(np.float32) * 0.05,
+        np.array([0.0, -0.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 3e38, 1e-40], dtype=np.float32),
+    ])
+    got = f32_to_bf16_bits(vals)
+    expect = torch.from_numpy(vals).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(got, expect)
