@@ -1,0 +1,287 @@
+"""Throughput of the D3Q19 collide-and-stream step on one MI355X.
+
+Workload: the periodic Taylor-Green vortex at 512^3 (D3Q19 BGK, tau = 0.56,
+u0 = 0.04), one step = one collideAndStream of the whole lattice.  The timed
+region runs with the populations already resident in HBM.  The headline value
+is GLUPS with FP32 population storage; the same step with FP16 and BF16
+shifted storage is timed beside it.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, ...).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X published HBM3E bandwidth
+BYTES_PER_POP = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--storages", default="f32,f16,bf16")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ens-storages", default="f32,f16")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB device copy (PMC counter calibration)")
+    return ap.parse_args()
+
+
+def time_storage(torch, size, storage, steps, warmup, block, tune=True):
+    from openlb_amd import Lattice, autotune, cases
+
+    dtype = np.float64 if storage == "f64" else np.float32
+    case = cases.tgv(size, u0=0.04, tau=0.56, dtype=dtype)
+    lat = Lattice(case.extent, case.padding, case.periodic, storage)
+    if block:
+        lat.set_launch(block)
+    case.apply(lat)
+    del case
+    stream = torch.cuda.current_stream()
+    handle = stream.cuda_stream
+    tuned = None
+    if tune and not block:
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+        tuned = {k: tuned[k] for k in ("block", "waves", "ms")}
+    lat.collide_and_stream(warmup, handle)
+    torch.cuda.synchronize()
+
+    # whole-job timing: K steps between two synchronisations
+    t0 = time.perf_counter()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    lat.collide_and_stream(steps, handle)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = start.elapsed_time(stop)
+
+    # per-launch kernel duration: HIP events around single steps on the stream
+    # the kernel is launched on
+    durs = []
+    for _ in range(min(steps, 20)):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        lat.collide_and_stream(1, handle)
+        b.record(stream)
+        durs.append((a, b))
+    torch.cuda.synchronize()
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in durs]))
+    moments = lat.reduce_moments(handle)
+    lat.close()
+    cells = size ** 3
+    return {
+        "ms_per_step": ms_total / steps,
+        "wall_s": wall,
+        "glups": cells * steps / (ms_total * 1e-3) / 1e9,
+        "kernel_ms": kernel_ms,
+        "kernel_glups": cells / (kernel_ms * 1e-3) / 1e9,
+        "bytes_per_cell": 2 * 19 * BYTES_PER_POP[storage],
+        "launch": tuned,
+        "finite": bool(np.isfinite(moments["mass"]) and np.isfinite(moments["sumUSqr"])),
+    }
+
+
+def time_ensemble(torch, storage, steps, warmup, tune=True):
+    """config 5: 64 lid-driven cavities, N ~ U{32..128}, U ~ U[0.02, 0.1],
+    Re = 100; persistent kernel per step (eager and graph-replayed) against
+    launching every cavity separately"""
+    from openlb_amd import Ensemble, Lattice, autotune, cases
+
+    dtype = np.float64 if storage == "f64" else np.float32
+    lats = []
+    for c in cases.ensemble_cavities(dtype=dtype):
+        lat = Lattice(c.extent, c.padding, c.periodic, storage)
+        c.apply(lat)
+        lats.append(lat)
+    ens = Ensemble(lats)
+    handle = torch.cuda.current_stream().cuda_stream
+    tuned = None
+    if tune:
+        tuned = autotune.tune_ensemble(ens, steps=3, stream=handle, use_cache=False)
+        tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "ms")}
+    ens.step(warmup, handle)
+    steps = max(10, steps // 10 * 10)
+    ms_persistent = ens.time("persistent", steps, handle)
+    ens.capture(10)
+    ens.launch_graph(1, handle)
+    ms_graph = ens.time("graph", steps, handle)
+    ms_sep = ens.time("separate", steps, handle)
+    cells = ens.cells
+    out = {
+        "members": len(lats),
+        "cells": cells,
+        "steps": steps,
+        "launch": tuned,
+        "graph_steps_per_s": steps / (ms_graph * 1e-3),
+        "persistent_steps_per_s": steps / (ms_persistent * 1e-3),
+        "separate_steps_per_s": steps / (ms_sep * 1e-3),
+        "graph_glups": cells * steps / (ms_graph * 1e-3) / 1e9,
+        "persistent_glups": cells * steps / (ms_persistent * 1e-3) / 1e9,
+        "separate_glups": cells * steps / (ms_sep * 1e-3) / 1e9,
+    }
+    out["speedup_graph_vs_separate"] = ms_sep / ms_graph
+    ens.close()
+    for lat in lats:
+        lat.close()
+    return out
+
+
+def copy_bandwidth(torch, nbytes=4 << 30, reps=10):
+    """measured device copy bandwidth (read + write bytes / time)"""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(size, steps):
+    """The reference's own CPU path (oracle/_ref, compiled from the OpenLB
+    sources) when it is present, else the C restatement in oracle/, timed on
+    all host cores on a bounded sample of the same workload."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "olb_ref_f")
+    cores = os.cpu_count() or 1
+    if os.path.exists(ref):
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
+        try:
+            out = subprocess.run([ref, "bench_tgv", str(size), str(steps)], env=env, capture_output=True,
+                                 text=True, timeout=300, check=True).stdout
+            mlups = float([ln for ln in out.splitlines() if ln.startswith("MLUPS")][-1].split()[1])
+            return {"value": mlups / 1000.0, "unit": "GLUPS", "cores": cores, "kind": "reference",
+                    "sample": f"OpenLB CPU_SIMD+OpenMP BGK TGV {size}^3 FP32, {steps} steps after 2 warm-up"}
+        except Exception as exc:  # fall through to the restatement
+            sys.stderr.write(f"reference baseline failed: {exc}\n")
+    from openlb_amd import cases
+    from oracle.oracle import OracleLattice, set_threads
+
+    set_threads(cores)
+    case = cases.tgv(size, dtype=np.float32)
+    lat = OracleLattice(case.extent, case.padding, case.periodic, "f32")
+    case.apply(lat)
+    lat.collide_and_stream(2)
+    t0 = time.perf_counter()
+    lat.collide_and_stream(steps)
+    dt = time.perf_counter() - t0
+    return {"value": size ** 3 * steps / dt / 1e9, "unit": "GLUPS", "cores": cores, "kind": "port",
+            "sample": f"C restatement (OpenMP) BGK TGV {size}^3 FP32, {steps} steps after 2 warm-up"}
+
+
+def load_traffic(storage):
+    path = os.path.join(ROOT, "profiles", f"traffic_step_{storage}_tgv512.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    if args.calib_copy:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": a.numel() * 4, "calib_copy_bytes_written": b.numel() * 4}))
+        return
+    storages = [s for s in args.storages.split(",") if s]
+    res = {}
+    for st in storages:
+        res[st] = time_storage(torch, args.size, st, args.steps, args.warmup, args.block, tune=not args.no_tune)
+        print(f"# {st}: {res[st]['glups']:.2f} GLUPS, kernel {res[st]['kernel_ms']:.3f} ms", file=sys.stderr)
+    ens = {}
+    if not args.no_ensemble:
+        for st in [x for x in args.ens_storages.split(",") if x]:
+            ens[st] = time_ensemble(torch, st, args.ens_steps, args.warmup, tune=not args.no_tune)
+            print(f"# ensemble {st}: graph {ens[st]['graph_steps_per_s']:.1f} steps/s "
+                  f"({ens[st]['graph_glups']:.2f} GLUPS), separate {ens[st]['separate_steps_per_s']:.1f} "
+                  f"steps/s, x{ens[st]['speedup_graph_vs_separate']:.2f}", file=sys.stderr)
+    copy_gbps = copy_bandwidth(torch)
+    head = res.get("f32") or next(iter(res.values()))
+    head_storage = "f32" if "f32" in res else storages[0]
+    cells = args.size ** 3
+    alg_bytes = head["bytes_per_cell"] * cells
+    achieved = alg_bytes / (head["kernel_ms"] * 1e-3) / 1e9
+    traffic = load_traffic(head_storage)
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": traffic,
+        "kernel": f"step_kernel<{head_storage}, BGK>",
+        "bytes_per_cell": head["bytes_per_cell"],
+        "measured_copy_GBps": round(copy_gbps, 1),
+        "frac_of_measured_copy": round(achieved / copy_gbps, 4),
+    }
+    base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_size, args.cpu_steps)
+    line = {
+        "metric": "D3Q19 GLUPS & %HBM roofline at 512^3 (FP32 vs FP16 store); ensemble steps/s",
+        "value": round(head["glups"], 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head_storage == "f32" else head_storage,
+        "data": "synthetic (analytic Taylor-Green initial field)",
+        "config": {"workload": f"taylor_green_d3q19_bgk_{args.size}^3", "cells": cells, "tau": 0.56,
+                   "u0": 0.04, "storage": head_storage, "parallelism": "single GPU"},
+        "storage_glups": {k: round(v["glups"], 3) for k, v in res.items()},
+        "storage_launch": {k: v["launch"] for k, v in res.items()},
+        "storage_hbm_frac": {k: round(v["bytes_per_cell"] * cells / (v["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+                             for k, v in res.items()},
+        "fp16_over_fp32": round(res["f16"]["glups"] / res["f32"]["glups"], 3) if "f16" in res and "f32" in res else None,
+        "ensemble": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                     for k, v in ens.items()},
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -151,9 +151,55 @@ olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* lattice, double out[8], void
 olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* lattice, int32_t value_type, void* rho, void* u);
 
 /* ---- launch configuration ---------------------------------------------------
- * threads per block of the step kernel (64..256, multiple of 64); 0 restores
- * the default.  Used by the autotuner. */
-olbgfx_status olbgfx_set_launch(olbgfx_lattice* lattice, uint32_t block_threads);
+ * threads per block of the step kernel (64..256, multiple of 64; 0 = 256) and
+ * an occupancy cap in waves per SIMD (1..8; 0 = as many as registers allow),
+ * enforced by reserving LDS per block.  Used by the autotuner; no reference
+ * counterpart (the reference fixes 64-thread blocks,
+ * core/platform/gpu/hip/operator.hh:426-433). */
+olbgfx_status olbgfx_set_launch(olbgfx_lattice* lattice, uint32_t block_threads,
+                                uint32_t waves_per_simd);
+
+/* ---- ensembles of independent lattices ------------------------------------
+ * A batch of lattices (same storage type, any extents and dynamics) advanced
+ * together: one persistent kernel per step walks a tile list that spans every
+ * member, and the ring offsets advance on the device so a captured hipGraph
+ * can be replayed.  Members stay owned by the caller and must outlive the
+ * ensemble.  No reference counterpart: OpenLB advances each
+ * SuperLattice::collideAndStream (core/superLattice.hh:663-725) on its own. */
+olbgfx_status olbgfx_ensemble_create(olbgfx_lattice* const* members, uint32_t count,
+                                     olbgfx_ensemble** out);
+olbgfx_status olbgfx_ensemble_destroy(olbgfx_ensemble* ensemble);
+
+/* threads per block (64..256), occupancy cap in waves per SIMD (0 = none) and
+ * tiles of block_threads cells per persistent block (>= 1); drops a captured
+ * graph.  Used by the autotuner. */
+olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* ensemble, uint32_t block_threads,
+                                         uint32_t waves_per_simd, uint32_t tiles_per_block);
+
+/* advance every member by `steps` steps, one persistent launch per step */
+olbgfx_status olbgfx_ensemble_step(olbgfx_ensemble* ensemble, uint32_t steps, void* stream);
+
+/* the comparison path: each member launched on its own, member after member
+ * (olbgfx_collide_and_stream per member and step) */
+olbgfx_status olbgfx_ensemble_step_separately(olbgfx_ensemble* ensemble, uint32_t steps,
+                                              void* stream);
+
+/* capture `steps_per_launch` ensemble steps into a hipGraph (synchronises);
+ * launch_graph replays it `launches` times on `stream`.  A change to any
+ * member's dynamics, mask or fields requires a new capture. */
+olbgfx_status olbgfx_ensemble_capture(olbgfx_ensemble* ensemble, uint32_t steps_per_launch);
+olbgfx_status olbgfx_ensemble_launch_graph(olbgfx_ensemble* ensemble, uint32_t launches,
+                                           void* stream);
+
+/* number of core cells over all members (cell updates per ensemble step) */
+olbgfx_status olbgfx_ensemble_cells(const olbgfx_ensemble* ensemble, uint64_t* cells);
+
+/* ---- timing (HIP events on `stream`; these synchronise) -------------------
+ * elapsed milliseconds of `steps` steps; for ensembles mode 0 = persistent
+ * kernel per step, 1 = replay of the captured graph, 2 = separate launches. */
+olbgfx_status olbgfx_time_steps(olbgfx_lattice* lattice, uint32_t steps, void* stream, float* ms);
+olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* ensemble, int32_t mode, uint32_t steps,
+                                   void* stream, float* ms);
 
 /* block until all work of this lattice is done */
 olbgfx_status olbgfx_synchronize(olbgfx_lattice* lattice);

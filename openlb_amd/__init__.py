@@ -101,8 +101,18 @@ def library() -> ctypes.CDLL:
         "olbgfx_get_statistics": [vp, ctypes.POINTER(ctypes.c_double)],
         "olbgfx_reduce_moments": [vp, ctypes.POINTER(ctypes.c_double), vp],
         "olbgfx_compute_rho_u": [vp, i32, vp, vp],
-        "olbgfx_set_launch": [vp, u32],
+        "olbgfx_set_launch": [vp, u32, u32],
         "olbgfx_synchronize": [vp],
+        "olbgfx_ensemble_create": [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)],
+        "olbgfx_ensemble_destroy": [vp],
+        "olbgfx_ensemble_set_launch": [vp, u32, u32, u32],
+        "olbgfx_ensemble_step": [vp, u32, vp],
+        "olbgfx_ensemble_step_separately": [vp, u32, vp],
+        "olbgfx_ensemble_capture": [vp, u32],
+        "olbgfx_ensemble_launch_graph": [vp, u32, vp],
+        "olbgfx_ensemble_cells": [vp, ctypes.POINTER(ctypes.c_uint64)],
+        "olbgfx_time_steps": [vp, u32, vp, ctypes.POINTER(ctypes.c_float)],
+        "olbgfx_ensemble_time": [vp, i32, u32, vp, ctypes.POINTER(ctypes.c_float)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -262,8 +272,16 @@ class Lattice:
     def synchronize(self) -> None:
         _check(library().olbgfx_synchronize(self._h))
 
-    def set_launch(self, block_threads: int) -> None:
-        _check(library().olbgfx_set_launch(self._h, int(block_threads)))
+    def time_steps(self, steps: int, stream: int | None = None) -> float:
+        """milliseconds of `steps` steps, HIP events on `stream` (synchronises)"""
+        ms = ctypes.c_float()
+        _check(library().olbgfx_time_steps(self._h, int(steps), stream, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def set_launch(self, block_threads: int, waves_per_simd: int = 0) -> None:
+        """threads per block (64..256) and an occupancy cap in waves per SIMD
+        (0 = no cap)"""
+        _check(library().olbgfx_set_launch(self._h, int(block_threads), int(waves_per_simd)))
 
     # -- statistics -----------------------------------------------------------------
     def set_statistics(self, on: bool) -> None:
@@ -299,3 +317,54 @@ class Lattice:
         u = np.empty(3 * self.ncells, dtype=self._dtype)
         _check(library().olbgfx_compute_rho_u(self._h, self._value, _ptr(rho), _ptr(u)))
         return rho, u.reshape(self.ncells, 3)
+
+
+class Ensemble:
+    """A batch of independent lattices advanced by one persistent kernel per
+    step (optionally replayed from a captured hipGraph)."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        _check(library().olbgfx_ensemble_create(arr, len(self.members), ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_uint64()
+        _check(library().olbgfx_ensemble_cells(self._h, ctypes.byref(n)))
+        self.cells = int(n.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            library().olbgfx_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, block_threads: int = 256, waves_per_simd: int = 0, tiles_per_block: int = 1) -> None:
+        _check(library().olbgfx_ensemble_set_launch(self._h, int(block_threads), int(waves_per_simd),
+                                                     int(tiles_per_block)))
+
+    def step(self, steps: int = 1, stream: int | None = None) -> None:
+        _check(library().olbgfx_ensemble_step(self._h, int(steps), stream))
+
+    def step_separately(self, steps: int = 1, stream: int | None = None) -> None:
+        _check(library().olbgfx_ensemble_step_separately(self._h, int(steps), stream))
+
+    def capture(self, steps_per_launch: int) -> None:
+        _check(library().olbgfx_ensemble_capture(self._h, int(steps_per_launch)))
+
+    def launch_graph(self, launches: int = 1, stream: int | None = None) -> None:
+        _check(library().olbgfx_ensemble_launch_graph(self._h, int(launches), stream))
+
+    def time(self, mode: str, steps: int, stream: int | None = None) -> float:
+        """milliseconds of `steps` ensemble steps; mode 'persistent', 'graph'
+        or 'separate' (HIP events on `stream`, synchronises)"""
+        code = {"persistent": 0, "graph": 1, "separate": 2}[mode]
+        ms = ctypes.c_float()
+        _check(library().olbgfx_ensemble_time(self._h, code, int(steps), stream, ctypes.byref(ms)))
+        return float(ms.value)
+

@@ -1,0 +1,102 @@
+"""Launch-shape autotuner for the step kernels.
+
+For a lattice it sweeps threads per block and an occupancy cap (waves per
+SIMD, enforced by an LDS reservation); for an ensemble also the number of
+tiles each persistent block walks.  Every candidate is timed with HIP events
+(olbgfx_time_steps / olbgfx_ensemble_time) after one untimed step, and the
+fastest is applied.  Results are cached in a JSON file keyed by shape,
+storage and dynamics mix, so a later run with the same key skips the sweep.
+
+The trial steps advance the lattice: tune during warm-up.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+BLOCKS = (64, 128, 256)
+WAVES = (0, 2, 3, 4, 5, 6)
+TILES = (1, 2, 4, 8)
+
+
+def _cache_path() -> str:
+    return os.environ.get("OLBGFX_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "olbgfx", "autotune.json"))
+
+
+def _load() -> dict:
+    try:
+        with open(_cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key: str, value: dict) -> None:
+    path = _cache_path()
+    data = _load()
+    data[key] = value
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path + ".tmp", "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(path + ".tmp", path)
+    except OSError:
+        pass
+
+
+def lattice_key(lat, mask_digest: str = "") -> str:
+    return f"lattice:{lat.extent}:{lat.padding}:{lat.periodic}:{lat.storage}:{mask_digest}"
+
+
+def tune_lattice(lat, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
+                 blocks=BLOCKS, waves=WAVES) -> dict:
+    """pick (block_threads, waves_per_simd) for one lattice and apply it"""
+    key = key or lattice_key(lat)
+    if use_cache:
+        hit = _load().get(key)
+        if hit:
+            lat.set_launch(hit["block"], hit["waves"])
+            return dict(hit, cached=True)
+    trials = []
+    for b in blocks:
+        for w in waves:
+            lat.set_launch(b, w)
+            lat.collide_and_stream(1, stream)
+            ms = lat.time_steps(steps, stream) / steps
+            trials.append({"block": b, "waves": w, "ms": ms})
+    best = min(trials, key=lambda t: t["ms"])
+    lat.set_launch(best["block"], best["waves"])
+    result = {"block": best["block"], "waves": best["waves"], "ms": best["ms"], "trials": trials}
+    _store(key, {"block": best["block"], "waves": best["waves"], "ms": best["ms"]})
+    return result
+
+
+def ensemble_key(extents, storage: str) -> str:
+    h = hashlib.sha1(repr(sorted(extents)).encode()).hexdigest()[:12]
+    return f"ensemble:{len(extents)}:{h}:{storage}"
+
+
+def tune_ensemble(ens, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
+                  blocks=BLOCKS, waves=WAVES, tiles=TILES) -> dict:
+    """pick (block_threads, waves_per_simd, tiles_per_block) for an ensemble"""
+    key = key or ensemble_key([m.extent for m in ens.members], ens.members[0].storage)
+    if use_cache:
+        hit = _load().get(key)
+        if hit:
+            ens.set_launch(hit["block"], hit["waves"], hit["tiles"])
+            return dict(hit, cached=True)
+    trials = []
+    for b in blocks:
+        for w in waves:
+            for t in tiles:
+                ens.set_launch(b, w, t)
+                ens.step(1, stream)
+                ms = ens.time("persistent", steps, stream) / steps
+                trials.append({"block": b, "waves": w, "tiles": t, "ms": ms})
+    best = min(trials, key=lambda t: t["ms"])
+    ens.set_launch(best["block"], best["waves"], best["tiles"])
+    _store(key, {k: best[k] for k in ("block", "waves", "tiles", "ms")})
+    return {"block": best["block"], "waves": best["waves"], "tiles": best["tiles"], "ms": best["ms"],
+            "trials": trials}

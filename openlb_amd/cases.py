@@ -284,3 +284,18 @@ def porous_channel(extent=(384, 128, 128), radius: float = 6.0, porosity: float 
               "tau": tau, "magic": magic, "force": force},
     )
 
+
+
+def ensemble_cavities(count: int = 64, nmin: int = 32, nmax: int = 128, umin: float = 0.02,
+                      umax: float = 0.1, re: float = 100.0, seed: int = 20261202, padding: int = 3,
+                      dtype=np.float32):
+    """Independent lid-driven cavities with edge lengths drawn uniformly from
+    {nmin..nmax} and lid velocities from U[umin, umax] (seeded splitmix64);
+    Re is held fixed through nu = U N / Re, tau = 3 nu + 1/2."""
+    rng = _SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = nmin + int(rng.next() % (nmax - nmin + 1))
+        u = umin + (umax - umin) * rng.uniform()
+        out.append(box_cavity(n, u_lid=u, re=re, padding=padding, dtype=dtype))
+    return out

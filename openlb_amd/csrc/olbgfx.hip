@@ -45,6 +45,8 @@ constexpr int Q = 19;
 // largest workgroup of the step kernel; more waves per block would cap the
 // registers per lane below what the D3Q19 collision needs without spilling
 constexpr uint32_t kMaxBlock = 256;
+// LDS of one gfx950 compute unit
+constexpr uint32_t kLdsPerCu = 160 * 1024;
 
 // D3Q19 velocity set, descriptor/definition/common.h:166-199.
 // Index 0 rests, 1..9 are the negative half, i+9 is the opposite of i.
@@ -94,13 +96,18 @@ struct Storage<double> {
   static __device__ __forceinline__ double unpack(double v) { return v; }
 };
 
+// FP16 storage in a named wrapper (so kernel names demangle readably)
+struct f16_t {
+  _Float16 v;
+};
+
 template <>
-struct Storage<_Float16> {
+struct Storage<f16_t> {
   using A = float;
   // v_cvt_f16_f32 rounds to nearest even and keeps FP16 subnormals
-  static __device__ __forceinline__ float load(const _Float16* p) { return (float)*p; }
-  static __device__ __forceinline__ _Float16 pack(float v) { return (_Float16)v; }
-  static __device__ __forceinline__ float unpack(_Float16 v) { return (float)v; }
+  static __device__ __forceinline__ float unpack(f16_t h) { return (float)h.v; }
+  static __device__ __forceinline__ float load(const f16_t* p) { return (float)p->v; }
+  static __device__ __forceinline__ f16_t pack(float v) { return f16_t{(_Float16)v}; }
 };
 
 template <>
@@ -519,7 +526,68 @@ __device__ __forceinline__ void ghost_all(void* pop, uint64_t stride, uint32_t c
 }
 
 // ---------------------------------------------------------------------------
-// the step kernel
+// one cell of the step: load, collide, store, periodic ghost writes
+
+template <typename S, int MODE>
+__device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>& a,
+                                          const uint32_t (&sh)[Q], uint32_t lin,
+                                          typename Storage<S>::A& st_rho,
+                                          typename Storage<S>::A& st_usqr, bool& counted)
+{
+  using A = typename Storage<S>::A;
+  using St = Storage<S>;
+  const uint32_t x = fast_div(lin, a.magic_nyz, a.nyz);
+  const uint32_t r = lin - x * a.nyz;
+  const uint32_t y = fast_div(r, a.magic_nz, (uint32_t)a.nz);
+  const uint32_t z = r - y * (uint32_t)a.nz;
+  const uint32_t n = ((x + a.pad) * a.py + (y + a.pad)) * a.pz + (z + a.pad);
+
+  int kind;
+  int id;
+  if constexpr (MODE >= 0) {
+    id = a.uniform_id;
+    kind = MODE;
+  } else {
+    id = (int)a.mask[n];
+    kind = a.table[id].kind;
+  }
+  if (kind == OLBGFX_NO_DYNAMICS) {
+    return;
+  }
+  A f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = St::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)));
+  }
+
+  if constexpr (MODE >= 0) {
+    counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
+  } else {
+    const DynEntry<A> e = a.table[id];
+    counted = apply_any(f, e, n, a, st_rho, st_usqr);
+  }
+
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
+  }
+
+  if (a.periodic) {
+    const bool lo[3] = {(a.periodic & 1) && x == 0, (a.periodic & 2) && y == 0,
+                        (a.periodic & 4) && z == 0};
+    const bool hi[3] = {(a.periodic & 1) && x == (uint32_t)a.nx - 1,
+                        (a.periodic & 2) && y == (uint32_t)a.ny - 1,
+                        (a.periodic & 4) && z == (uint32_t)a.nz - 1};
+    if (lo[0] | lo[1] | lo[2] | hi[0] | hi[1] | hi[2]) {
+      const int64_t off[3] = {(int64_t)a.nx * a.py * a.pz, (int64_t)a.ny * a.pz, (int64_t)a.nz};
+      ghost_all<S>(a.pop, a.stride, a.count, sh, (int64_t)n, lo, hi, off, f,
+                   std::make_integer_sequence<int, Q - 1>{});
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the step kernel of one lattice
 //
 // MODE < 0: every cell reads its dynamics id from the mask;
 // MODE = k: every core cell has the dynamics of table slot uniform_id, whose
@@ -529,7 +597,6 @@ template <typename S, int MODE, bool STATS>
 __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename Storage<S>::A> a)
 {
   using A = typename Storage<S>::A;
-  using St = Storage<S>;
 
   // XCD-aware order: the hardware deals consecutive workgroups round-robin to
   // the 8 XCDs; map them so each XCD sweeps one contiguous part of the block
@@ -551,56 +618,8 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
 
   A st_rho = A(0), st_usqr = A(0);
   bool counted = false;
-
   if (active) {
-    const uint32_t x = fast_div(lin, a.magic_nyz, a.nyz);
-    const uint32_t r = lin - x * a.nyz;
-    const uint32_t y = fast_div(r, a.magic_nz, (uint32_t)a.nz);
-    const uint32_t z = r - y * (uint32_t)a.nz;
-    const uint32_t n = ((x + a.pad) * a.py + (y + a.pad)) * a.pz + (z + a.pad);
-
-    int kind;
-    int id;
-    if constexpr (MODE >= 0) {
-      id = a.uniform_id;
-      kind = MODE;
-    } else {
-      id = (int)a.mask[n];
-      kind = a.table[id].kind;
-    }
-    if (kind != OLBGFX_NO_DYNAMICS) {
-      A f[Q];
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        f[i] = St::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)));
-      }
-
-      if constexpr (MODE >= 0) {
-        counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
-      } else {
-        const DynEntry<A> e = a.table[id];
-        counted = apply_any(f, e, n, a, st_rho, st_usqr);
-      }
-
-#pragma unroll
-      for (int i = 0; i < Q; ++i) {
-        *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
-      }
-
-      if (a.periodic) {
-        const bool lo[3] = {(a.periodic & 1) && x == 0, (a.periodic & 2) && y == 0,
-                            (a.periodic & 4) && z == 0};
-        const bool hi[3] = {(a.periodic & 1) && x == (uint32_t)a.nx - 1,
-                            (a.periodic & 2) && y == (uint32_t)a.ny - 1,
-                            (a.periodic & 4) && z == (uint32_t)a.nz - 1};
-        if (lo[0] | lo[1] | lo[2] | hi[0] | hi[1] | hi[2]) {
-          const int64_t off[3] = {(int64_t)a.nx * a.py * a.pz, (int64_t)a.ny * a.pz,
-                                  (int64_t)a.nz};
-          ghost_all<S>(a.pop, a.stride, a.count, sh, (int64_t)n, lo, hi, off, f,
-                       std::make_integer_sequence<int, Q - 1>{});
-        }
-      }
-    }
+    step_cell<S, MODE>(a, sh, lin, st_rho, st_usqr, counted);
   }
 
   if constexpr (STATS) {
@@ -608,6 +627,63 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
     const double r = counted ? (double)st_rho : 0.0;
     const double u2 = counted ? (double)st_usqr : 0.0;
     block_reduce4(c, r, u2, u2, a.stats + 4 * (size_t)lb, in_grid);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the ensemble kernel: one persistent launch advances every member lattice by
+// one step.  The work is a list of tiles (member, first core cell) of
+// blockDim.x cells each; logical block lb owns the contiguous tile range
+// [lb*tiles_per_block, (lb+1)*tiles_per_block), with the XCD-aware order of
+// the single-lattice kernel.  Member descriptors and ring offsets live in
+// device memory, so the launch has fixed arguments and can be replayed from a
+// captured graph.
+
+struct Tile {
+  uint32_t member;
+  uint32_t first;
+};
+
+template <typename S>
+__global__ void __launch_bounds__(kMaxBlock)
+    ensemble_kernel(const StepArgs<typename Storage<S>::A>* members, const Tile* tiles,
+                    uint32_t ntiles, uint32_t tiles_per_block, uint32_t nblocks,
+                    uint32_t blocks_per_xcd)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b & 7u) * blocks_per_xcd + (b >> 3);
+  if (lb >= nblocks) {
+    return;
+  }
+  const uint32_t t0 = lb * tiles_per_block;
+  const uint32_t t1 = min(t0 + tiles_per_block, ntiles);
+  for (uint32_t t = t0; t < t1; ++t) {
+    const Tile tile = tiles[t];
+    const StepArgs<A>& a = members[tile.member];
+    uint32_t sh[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) sh[i] = a.shift_dev[i];
+    const uint32_t lin = tile.first + threadIdx.x;
+    A st_rho, st_usqr;
+    bool counted = false;
+    if (lin < a.ncore) {
+      step_cell<S, -1>(a, sh, lin, st_rho, st_usqr, counted);
+    }
+  }
+}
+
+// advance the device ring offsets of `count` lattices by one step:
+// s_i -= d_i (mod ncells); dmod holds (-d_i) mod ncells per lattice
+__global__ void advance_shifts(uint32_t* const* shifts, const uint32_t* dmod, const uint32_t* counts,
+                               uint32_t nlat)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nlat * Q) {
+    const uint32_t l = k / Q, i = k % Q;
+    uint32_t* sh = shifts[l];
+    const uint64_t s = (uint64_t)sh[i] + dmod[k];
+    sh[i] = (uint32_t)(s >= counts[l] ? s - counts[l] : s);
   }
 }
 
@@ -623,16 +699,6 @@ __global__ void __launch_bounds__(256) stats_finalize(const double* partial, uin
     m = fmax(m, partial[4 * (size_t)k + 3]);
   }
   block_reduce4(c, r, e, m, out, true);
-}
-
-// advance the device offsets by one step: s_i -= d_i (mod count)
-__global__ void advance_shifts(uint32_t* shift, const int64_t* dmod, uint32_t count)
-{
-  const int i = threadIdx.x;
-  if (i < Q) {
-    const uint64_t s = (uint64_t)shift[i] + (uint64_t)dmod[i];
-    shift[i] = (uint32_t)(s >= count ? s - count : s);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -815,7 +881,6 @@ struct olbgfx_lattice {
   void* pop = nullptr;
   uint8_t* mask = nullptr;
   uint32_t* shift_dev = nullptr;
-  int64_t* dmod_dev = nullptr;
   int32_t* kinds_dev = nullptr;
   void* cell_u = nullptr;
   void* cell_rho = nullptr;
@@ -827,8 +892,11 @@ struct olbgfx_lattice {
   bool stats_valid = false;
   int32_t uniform_id = 0;     // all cells start with id 0 = NoDynamics
   uint32_t block_threads = 256;
+  uint32_t lds_reserve = 0;    // dynamic LDS per block that limits occupancy
   olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
   std::vector<uint8_t> host_mask;
+  uint64_t version = 0;        // bumped by every change of dynamics, mask or fields
+  bool shift_dev_valid = true; // device copy of the offsets matches the host mirror
 };
 
 namespace {
@@ -889,26 +957,26 @@ olbgfx_status fill_args(olbgfx_lattice* L, StepArgs<A>& a, bool device_shifts)
 }
 
 template <typename S, bool STATS>
-void launch_mode(int mode, dim3 grid, dim3 block, hipStream_t stream,
+void launch_mode(int mode, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
                  const StepArgs<typename Storage<S>::A>& a)
 {
   switch (mode) {
   case OLBGFX_NO_DYNAMICS:
     break;
   case OLBGFX_BGK:
-    step_kernel<S, OLBGFX_BGK, STATS><<<grid, block, 0, stream>>>(a);
+    step_kernel<S, OLBGFX_BGK, STATS><<<grid, block, lds, stream>>>(a);
     break;
   case OLBGFX_TRT:
-    step_kernel<S, OLBGFX_TRT, STATS><<<grid, block, 0, stream>>>(a);
+    step_kernel<S, OLBGFX_TRT, STATS><<<grid, block, lds, stream>>>(a);
     break;
   case OLBGFX_FORCED_BGK:
-    step_kernel<S, OLBGFX_FORCED_BGK, STATS><<<grid, block, 0, stream>>>(a);
+    step_kernel<S, OLBGFX_FORCED_BGK, STATS><<<grid, block, lds, stream>>>(a);
     break;
   case OLBGFX_FORCED_TRT:
-    step_kernel<S, OLBGFX_FORCED_TRT, STATS><<<grid, block, 0, stream>>>(a);
+    step_kernel<S, OLBGFX_FORCED_TRT, STATS><<<grid, block, lds, stream>>>(a);
     break;
   default:
-    step_kernel<S, -1, STATS><<<grid, block, 0, stream>>>(a);
+    step_kernel<S, -1, STATS><<<grid, block, lds, stream>>>(a);
     break;
   }
 }
@@ -937,10 +1005,10 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
   for (uint32_t k = 0; k < steps; ++k) {
     for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
     if (L->stats_on) {
-      launch_mode<S, true>(mode, grid, block, stream, a);
+      launch_mode<S, true>(mode, grid, block, L->lds_reserve, stream, a);
       stats_finalize<<<1, 256, 0, stream>>>(L->stats_partial, a.nblocks, L->stats_out);
     } else {
-      launch_mode<S, false>(mode, grid, block, stream, a);
+      launch_mode<S, false>(mode, grid, block, L->lds_reserve, stream, a);
     }
     HIP_TRY(hipGetLastError());
     // stream(): s_i -= d_i (mod count), core/blockLattice.hh:316-338
@@ -951,6 +1019,7 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
   }
   if (steps > 0) {
     L->stats_valid = L->stats_on;
+    L->shift_dev_valid = false;
   }
   return OLBGFX_OK;
 }
@@ -958,6 +1027,7 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
 olbgfx_status sync_shifts(olbgfx_lattice* L)
 {
   HIP_TRY(hipMemcpy(L->shift_dev, L->shift, sizeof(L->shift), hipMemcpyHostToDevice));
+  L->shift_dev_valid = true;
   return OLBGFX_OK;
 }
 
@@ -1012,7 +1082,7 @@ olbgfx_status dispatch_storage(olbgfx_lattice* L, F&& f)
   case OLBGFX_STORE_F32:
     return f((float*)nullptr);
   case OLBGFX_STORE_F16:
-    return f((_Float16*)nullptr);
+    return f((f16_t*)nullptr);
   case OLBGFX_STORE_BF16:
     return f((bf16_t*)nullptr);
   case OLBGFX_STORE_F64:
@@ -1156,8 +1226,6 @@ olbgfx_status olbgfx_lattice_create(const olbgfx_lattice_desc* desc, olbgfx_latt
       hipMalloc(&L->mask, count) != hipSuccess || hipMemset(L->mask, 0, count) != hipSuccess ||
       hipMalloc(&L->shift_dev, sizeof(L->shift)) != hipSuccess ||
       hipMemset(L->shift_dev, 0, sizeof(L->shift)) != hipSuccess ||
-      hipMalloc(&L->dmod_dev, sizeof(L->dmod)) != hipSuccess ||
-      hipMemcpy(L->dmod_dev, L->dmod, sizeof(L->dmod), hipMemcpyHostToDevice) != hipSuccess ||
       hipMalloc(&L->kinds_dev, sizeof(int32_t) * 256) != hipSuccess ||
       hipMemset(L->kinds_dev, 0, sizeof(int32_t) * 256) != hipSuccess ||
       hipMalloc(&L->stats_out, 4 * sizeof(double)) != hipSuccess ||
@@ -1177,7 +1245,6 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->pop);
   hipFree(L->mask);
   hipFree(L->shift_dev);
-  hipFree(L->dmod_dev);
   hipFree(L->kinds_dev);
   hipFree(L->cell_u);
   hipFree(L->cell_rho);
@@ -1218,6 +1285,9 @@ olbgfx_status olbgfx_ring_shifts(olbgfx_lattice* L, int64_t out[19])
 
 olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_dynamics_desc* dyn)
 {
+  if (L) {
+    ++L->version;
+  }
   if (check(L) != OLBGFX_OK || !dyn) {
     return fail(OLBGFX_ERR_INVALID, "null argument");
   }
@@ -1236,6 +1306,9 @@ olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_d
 
 olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* L, const uint8_t* ids)
 {
+  if (L) {
+    ++L->version;
+  }
   if (check(L) != OLBGFX_OK || !ids) {
     return fail(OLBGFX_ERR_INVALID, "null argument");
   }
@@ -1252,6 +1325,9 @@ olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* L, const uint8_t* ids)
 
 olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* L, int32_t field, int32_t vt, const void* values)
 {
+  if (L) {
+    ++L->version;
+  }
   if (check(L) != OLBGFX_OK) {
     return OLBGFX_ERR_INVALID;
   }
@@ -1476,8 +1552,11 @@ olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* L, int32_t vt, void* rho, voi
   return st;
 }
 
-olbgfx_status olbgfx_set_launch(olbgfx_lattice* L, uint32_t block_threads)
+olbgfx_status olbgfx_set_launch(olbgfx_lattice* L, uint32_t block_threads, uint32_t waves_per_simd)
 {
+  if (L) {
+    ++L->version;
+  }
   if (check(L) != OLBGFX_OK) {
     return OLBGFX_ERR_INVALID;
   }
@@ -1488,7 +1567,23 @@ olbgfx_status olbgfx_set_launch(olbgfx_lattice* L, uint32_t block_threads)
   if (block_threads < 64 || block_threads > kMaxBlock || block_threads % 64 != 0) {
     return fail(OLBGFX_ERR_INVALID, "block size must be a multiple of 64 in [64, 256]");
   }
+  if (waves_per_simd > 8) {
+    return fail(OLBGFX_ERR_INVALID, "waves per SIMD must be in [0, 8]");
+  }
+  uint32_t lds = 0;
+  if (waves_per_simd > 0) {
+    // a CU has 4 SIMDs and 160 KiB of LDS: reserving LDS per block caps the
+    // blocks that fit on one CU, and with them the waves per SIMD
+    const uint32_t waves_per_block = block_threads / 64;
+    const uint32_t blocks_per_cu = std::max(1u, 4 * waves_per_simd / waves_per_block);
+    lds = kLdsPerCu / blocks_per_cu / 1024 * 1024;
+    if (lds * (blocks_per_cu + 1) <= kLdsPerCu) {
+      lds += 1024;
+    }
+    lds = std::min(lds, kLdsPerCu - 4096);  // room for the kernel's own reduction scratch
+  }
   L->block_threads = block_threads;
+  L->lds_reserve = lds;
   return OLBGFX_OK;
 }
 
@@ -1499,6 +1594,424 @@ olbgfx_status olbgfx_synchronize(olbgfx_lattice* L)
   }
   HIP_TRY(hipDeviceSynchronize());
   return OLBGFX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ensemble of independent lattices
+
+struct olbgfx_ensemble {
+  std::vector<olbgfx_lattice*> members;
+  int32_t storage = OLBGFX_STORE_F32;
+  void* args_dev = nullptr;          // StepArgs<A>[members]
+  Tile* tiles_dev = nullptr;
+  uint32_t ntiles = 0;
+  uint32_t tiles_capacity = 0;
+  uint32_t** shift_ptrs_dev = nullptr;
+  uint32_t* dmod_dev = nullptr;
+  uint32_t* counts_dev = nullptr;
+  uint32_t block_threads = 256;
+  uint32_t waves_per_simd = 0;
+  uint32_t tiles_per_block = 1;
+  size_t lds = 0;
+  std::vector<uint64_t> versions;
+  uint32_t tiled_block = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture_stream = nullptr;
+  uint32_t graph_steps = 0;
+};
+
+namespace {
+
+size_t lds_for(uint32_t block_threads, uint32_t waves_per_simd)
+{
+  if (waves_per_simd == 0) {
+    return 0;
+  }
+  const uint32_t waves_per_block = block_threads / 64;
+  const uint32_t blocks_per_cu = std::max(1u, 4 * waves_per_simd / waves_per_block);
+  uint32_t lds = kLdsPerCu / blocks_per_cu / 1024 * 1024;
+  if (lds * (blocks_per_cu + 1) <= kLdsPerCu) {
+    lds += 1024;
+  }
+  return std::min(lds, kLdsPerCu - 4096);
+}
+
+// host mirror of the offsets after `steps` more steps
+void advance_host_shifts(olbgfx_lattice* L, uint64_t steps)
+{
+  const uint64_t k = steps % L->count;
+  for (int i = 0; i < Q; ++i) {
+    const uint64_t add = (k * (uint64_t)L->dmod[i]) % L->count;
+    const uint64_t s = (uint64_t)L->shift[i] + add;
+    L->shift[i] = (uint32_t)(s >= L->count ? s - L->count : s);
+  }
+}
+
+// (re)build the device descriptors and the tile list of the ensemble
+template <typename S>
+olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
+{
+  using A = typename Storage<S>::A;
+  const size_t m = E->members.size();
+  bool changed = force || E->tiled_block != E->block_threads;
+  for (size_t k = 0; k < m; ++k) {
+    changed = changed || E->versions[k] != E->members[k]->version;
+  }
+  for (size_t k = 0; k < m; ++k) {
+    if (!E->members[k]->shift_dev_valid) {
+      if (sync_shifts(E->members[k]) != OLBGFX_OK) {
+        return OLBGFX_ERR_HIP;
+      }
+    }
+  }
+  if (!changed) {
+    return OLBGFX_OK;
+  }
+  std::vector<StepArgs<A>> args(m);
+  std::vector<Tile> tiles;
+  std::vector<uint32_t*> shift_ptrs(m);
+  std::vector<uint32_t> dmod(m * Q), counts(m);
+  for (size_t k = 0; k < m; ++k) {
+    olbgfx_lattice* L = E->members[k];
+    fill_args(L, args[k], true);
+    args[k].uniform_id = -1;
+    args[k].stats = nullptr;
+    shift_ptrs[k] = L->shift_dev;
+    counts[k] = L->count;
+    for (int i = 0; i < Q; ++i) dmod[k * Q + i] = (uint32_t)L->dmod[i];
+    for (uint32_t first = 0; first < args[k].ncore; first += E->block_threads) {
+      tiles.push_back(Tile{(uint32_t)k, first});
+    }
+    E->versions[k] = L->version;
+  }
+  if (!E->args_dev) {
+    HIP_TRY(hipMalloc(&E->args_dev, sizeof(StepArgs<A>) * m));
+    HIP_TRY(hipMalloc(&E->shift_ptrs_dev, sizeof(uint32_t*) * m));
+    HIP_TRY(hipMalloc(&E->dmod_dev, sizeof(uint32_t) * m * Q));
+    HIP_TRY(hipMalloc(&E->counts_dev, sizeof(uint32_t) * m));
+  }
+  if (tiles.size() > E->tiles_capacity) {
+    if (E->tiles_dev) {
+      HIP_TRY(hipFree(E->tiles_dev));
+    }
+    HIP_TRY(hipMalloc(&E->tiles_dev, sizeof(Tile) * tiles.size()));
+    E->tiles_capacity = (uint32_t)tiles.size();
+  }
+  HIP_TRY(hipMemcpy(E->args_dev, args.data(), sizeof(StepArgs<A>) * m, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(E->tiles_dev, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(E->shift_ptrs_dev, shift_ptrs.data(), sizeof(uint32_t*) * m, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(E->dmod_dev, dmod.data(), sizeof(uint32_t) * m * Q, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(E->counts_dev, counts.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+  E->ntiles = (uint32_t)tiles.size();
+  E->tiled_block = E->block_threads;
+  return OLBGFX_OK;
+}
+
+// enqueue `steps` ensemble steps (persistent kernel + offset advance)
+template <typename S>
+olbgfx_status ensemble_enqueue(olbgfx_ensemble* E, uint32_t steps, hipStream_t stream)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t nblocks = (E->ntiles + E->tiles_per_block - 1) / E->tiles_per_block;
+  const uint32_t per_xcd = (nblocks + 7) / 8;
+  const dim3 grid(8 * per_xcd), block(E->block_threads);
+  const uint32_t m = (uint32_t)E->members.size();
+  const dim3 agrid((m * Q + 255) / 256), ablock(256);
+  for (uint32_t k = 0; k < steps; ++k) {
+    ensemble_kernel<S><<<grid, block, E->lds, stream>>>((const StepArgs<A>*)E->args_dev, E->tiles_dev,
+                                                        E->ntiles, E->tiles_per_block, nblocks,
+                                                        per_xcd);
+    advance_shifts<<<agrid, ablock, 0, stream>>>(E->shift_ptrs_dev, E->dmod_dev, E->counts_dev, m);
+  }
+  HIP_TRY(hipGetLastError());
+  return OLBGFX_OK;
+}
+
+template <typename F>
+olbgfx_status dispatch_ensemble(olbgfx_ensemble* E, F&& f)
+{
+  switch (E->storage) {
+  case OLBGFX_STORE_F32:
+    return f((float*)nullptr);
+  case OLBGFX_STORE_F16:
+    return f((f16_t*)nullptr);
+  case OLBGFX_STORE_BF16:
+    return f((bf16_t*)nullptr);
+  case OLBGFX_STORE_F64:
+    return f((double*)nullptr);
+  default:
+    return fail(OLBGFX_ERR_INVALID, "unknown storage type");
+  }
+}
+
+void drop_graph(olbgfx_ensemble* E)
+{
+  if (E->exec) {
+    (void)hipGraphExecDestroy(E->exec);
+    E->exec = nullptr;
+  }
+  if (E->graph) {
+    (void)hipGraphDestroy(E->graph);
+    E->graph = nullptr;
+  }
+  E->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_ensemble_create(olbgfx_lattice* const* members, uint32_t count,
+                                     olbgfx_ensemble** out)
+{
+  if (!members || !out || count == 0) {
+    return fail(OLBGFX_ERR_INVALID, "an ensemble needs at least one member");
+  }
+  *out = nullptr;
+  const int32_t storage = members[0] ? members[0]->desc.storage : -1;
+  for (uint32_t k = 0; k < count; ++k) {
+    if (!members[k]) {
+      return fail(OLBGFX_ERR_INVALID, "null member lattice");
+    }
+    if (members[k]->desc.storage != storage) {
+      return fail(OLBGFX_ERR_INVALID, "all members of an ensemble need the same storage type");
+    }
+  }
+  olbgfx_ensemble* E = new (std::nothrow) olbgfx_ensemble();
+  if (!E) {
+    return fail(OLBGFX_ERR_NOMEM, "host allocation failed");
+  }
+  E->members.assign(members, members + count);
+  E->versions.assign(count, ~0ull);
+  E->storage = storage;
+  *out = E;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_destroy(olbgfx_ensemble* E)
+{
+  if (!E) {
+    return OLBGFX_OK;
+  }
+  (void)hipDeviceSynchronize();
+  drop_graph(E);
+  if (E->capture_stream) {
+    (void)hipStreamDestroy(E->capture_stream);
+  }
+  (void)hipFree(E->args_dev);
+  (void)hipFree(E->tiles_dev);
+  (void)hipFree(E->shift_ptrs_dev);
+  (void)hipFree(E->dmod_dev);
+  (void)hipFree(E->counts_dev);
+  delete E;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* E, uint32_t block_threads,
+                                         uint32_t waves_per_simd, uint32_t tiles_per_block)
+{
+  if (!E) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble");
+  }
+  if (block_threads == 0) {
+    block_threads = 256;
+  }
+  if (block_threads < 64 || block_threads > kMaxBlock || block_threads % 64 != 0) {
+    return fail(OLBGFX_ERR_INVALID, "block size must be a multiple of 64 in [64, 256]");
+  }
+  if (waves_per_simd > 8 || tiles_per_block == 0 || tiles_per_block > 4096) {
+    return fail(OLBGFX_ERR_INVALID, "waves per SIMD in [0, 8], tiles per block in [1, 4096]");
+  }
+  E->block_threads = block_threads;
+  E->waves_per_simd = waves_per_simd;
+  E->tiles_per_block = tiles_per_block;
+  E->lds = lds_for(block_threads, waves_per_simd);
+  drop_graph(E);
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_step(olbgfx_ensemble* E, uint32_t steps, void* stream)
+{
+  if (!E) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  olbgfx_status st = dispatch_ensemble(E, [&](auto* tag) -> olbgfx_status {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    if (ensemble_prepare<S>(E, false) != OLBGFX_OK) {
+      return OLBGFX_ERR_HIP;
+    }
+    return ensemble_enqueue<S>(E, steps, s);
+  });
+  if (st == OLBGFX_OK) {
+    for (olbgfx_lattice* L : E->members) {
+      advance_host_shifts(L, steps);
+      L->stats_valid = false;
+    }
+  }
+  return st;
+}
+
+olbgfx_status olbgfx_ensemble_step_separately(olbgfx_ensemble* E, uint32_t steps, void* stream)
+{
+  if (!E) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble");
+  }
+  for (uint32_t k = 0; k < steps; ++k) {
+    for (olbgfx_lattice* L : E->members) {
+      const olbgfx_status st = olbgfx_collide_and_stream(L, 1, stream);
+      if (st != OLBGFX_OK) {
+        return st;
+      }
+    }
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_capture(olbgfx_ensemble* E, uint32_t steps_per_launch)
+{
+  if (!E || steps_per_launch == 0) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble or zero steps");
+  }
+  drop_graph(E);
+  if (!E->capture_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking));
+  }
+  olbgfx_status st = dispatch_ensemble(E, [&](auto* tag) -> olbgfx_status {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    if (ensemble_prepare<S>(E, false) != OLBGFX_OK) {
+      return OLBGFX_ERR_HIP;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipStreamBeginCapture(E->capture_stream, hipStreamCaptureModeThreadLocal));
+    const olbgfx_status q = ensemble_enqueue<S>(E, steps_per_launch, E->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(E->capture_stream, &g);
+    if (q != OLBGFX_OK) {
+      if (g) {
+        (void)hipGraphDestroy(g);
+      }
+      return q;
+    }
+    if (end != hipSuccess) {
+      return fail(OLBGFX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(end));
+    }
+    E->graph = g;
+    HIP_TRY(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
+    return OLBGFX_OK;
+  });
+  if (st == OLBGFX_OK) {
+    E->graph_steps = steps_per_launch;
+  } else {
+    drop_graph(E);
+  }
+  return st;
+}
+
+olbgfx_status olbgfx_ensemble_launch_graph(olbgfx_ensemble* E, uint32_t launches, void* stream)
+{
+  if (!E || !E->exec) {
+    return fail(OLBGFX_ERR_INVALID, "no captured graph (call olbgfx_ensemble_capture)");
+  }
+  for (size_t k = 0; k < E->members.size(); ++k) {
+    if (E->versions[k] != E->members[k]->version || !E->members[k]->shift_dev_valid) {
+      return fail(OLBGFX_ERR_INVALID, "a member changed since the capture; capture again");
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (uint32_t k = 0; k < launches; ++k) {
+    HIP_TRY(hipGraphLaunch(E->exec, s));
+  }
+  for (olbgfx_lattice* L : E->members) {
+    advance_host_shifts(L, (uint64_t)launches * E->graph_steps);
+    L->stats_valid = false;
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_cells(const olbgfx_ensemble* E, uint64_t* cells)
+{
+  if (!E || !cells) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  uint64_t c = 0;
+  for (const olbgfx_lattice* L : E->members) {
+    c += (uint64_t)L->desc.nx * L->desc.ny * L->desc.nz;
+  }
+  *cells = c;
+  return OLBGFX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// timing helpers for the autotuner and benchmarks
+
+namespace {
+
+template <typename F>
+olbgfx_status time_on_stream(hipStream_t s, F&& enqueue, float* ms)
+{
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(OLBGFX_ERR_HIP, "hipEventCreate failed");
+  }
+  olbgfx_status st = OLBGFX_OK;
+  if (hipEventRecord(a, s) != hipSuccess) {
+    st = fail(OLBGFX_ERR_HIP, "hipEventRecord failed");
+  }
+  if (st == OLBGFX_OK) {
+    st = enqueue();
+  }
+  if (st == OLBGFX_OK &&
+      (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+       hipEventElapsedTime(ms, a, b) != hipSuccess)) {
+    st = fail(OLBGFX_ERR_HIP, "event timing failed");
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_time_steps(olbgfx_lattice* L, uint32_t steps, void* stream, float* ms)
+{
+  if (check(L) != OLBGFX_OK || !ms) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return time_on_stream(s, [&] { return olbgfx_collide_and_stream(L, steps, stream); }, ms);
+}
+
+olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* E, int32_t mode, uint32_t steps, void* stream,
+                                   float* ms)
+{
+  if (!E || !ms) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (mode) {
+  case 0:
+    return time_on_stream(s, [&] { return olbgfx_ensemble_step(E, steps, stream); }, ms);
+  case 1:
+    if (!E->exec || E->graph_steps == 0 || steps % E->graph_steps != 0) {
+      return fail(OLBGFX_ERR_INVALID, "graph mode needs a capture whose steps divide `steps`");
+    }
+    return time_on_stream(
+        s, [&] { return olbgfx_ensemble_launch_graph(E, steps / E->graph_steps, stream); }, ms);
+  case 2:
+    return time_on_stream(s, [&] { return olbgfx_ensemble_step_separately(E, steps, stream); }, ms);
+  default:
+    return fail(OLBGFX_ERR_INVALID, "mode: 0 persistent, 1 graph, 2 separate launches");
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,135 @@
+"""Ensemble path: one persistent kernel per step over many lattices, eager
+and replayed from a captured hipGraph.  The per-cell arithmetic is the one of
+the single-lattice kernel, so results are compared bit for bit with the
+members stepped one by one (exact for integer state and populations)."""
+import numpy as np
+import pytest
+
+from openlb_amd import BGK, Dynamics, Ensemble, Lattice, OlbgfxError, autotune, cases
+from oracle.oracle import OracleLattice
+from tests.helpers import rel_err_unshifted
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [6, 9, 12, 7, 15]
+
+
+def _members(storage, sizes=SIZES):
+    dtype = np.float64 if storage == "f64" else np.float32
+    cs = [cases.box_cavity(n, u_lid=0.02 + 0.01 * k, dtype=dtype) for k, n in enumerate(sizes)]
+    lats = []
+    for c in cs:
+        lat = Lattice(c.extent, c.padding, c.periodic, storage)
+        c.apply(lat)
+        lats.append(lat)
+    return cs, lats
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
+def test_ensemble_equals_members_stepped_alone(storage):
+    print(f"[ensemble] storage={storage}", flush=True)
+    _, a = _members(storage)
+    _, b = _members(storage)
+    ens = Ensemble(a)
+    ens.step(20)
+    for lat in b:
+        lat.collide_and_stream(20)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+        np.testing.assert_array_equal(x.ring_shifts(), y.ring_shifts())
+
+
+def test_graph_replay_equals_eager_steps():
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ea, eb = Ensemble(a), Ensemble(b)
+    ea.capture(4)
+    ea.launch_graph(5)
+    eb.step(20)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+
+
+def test_graph_with_tuned_launch_shape():
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ea = Ensemble(a)
+    ea.set_launch(128, 3, 4)
+    ea.capture(3)
+    ea.launch_graph(4)
+    for lat in b:
+        lat.collide_and_stream(12)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+
+
+def test_interleaved_ensemble_and_single_steps():
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ens = Ensemble(a)
+    ens.step(5)
+    a[2].collide_and_stream(3)
+    ens.step(4)
+    ens.step_separately(2)
+    ens.step(1)
+    for k, lat in enumerate(b):
+        lat.collide_and_stream(12 + (3 if k == 2 else 0))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+
+
+def test_ensemble_member_matches_oracle():
+    cs, a = _members("f32", sizes=[10, 14])
+    Ensemble(a).step(40)
+    for c, lat in zip(cs, a):
+        orc = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+        c.apply(orc)
+        orc.collide_and_stream(40)
+        assert rel_err_unshifted(lat.get_populations(), orc.get_populations()) <= 1e-5
+
+
+def test_periodic_member_in_ensemble():
+    """a uniform periodic lattice runs the masked path inside the ensemble
+    and the uniform path alone: both agree to FP32 rounding"""
+    c = cases.tgv(12, dtype=np.float32)
+    x = Lattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(x)
+    y = Lattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(y)
+    Ensemble([x]).step(15)
+    y.collide_and_stream(15)
+    core = c.core_mask().reshape(-1)
+    assert rel_err_unshifted(x.get_populations()[:, core], y.get_populations()[:, core]) <= 1e-6
+
+
+def test_graph_refuses_changed_members():
+    _, a = _members("f32", sizes=[6, 8])
+    ens = Ensemble(a)
+    ens.capture(2)
+    a[0].set_dynamics(4, Dynamics(BGK, omega=1.1))
+    with pytest.raises(OlbgfxError):
+        ens.launch_graph(1)
+    ens.capture(2)
+    ens.launch_graph(1)
+
+
+def test_mixed_storage_is_rejected():
+    a = Lattice((4, 4, 4), padding=1, storage="f32")
+    b = Lattice((4, 4, 4), padding=1, storage="f16")
+    with pytest.raises(OlbgfxError):
+        Ensemble([a, b])
+
+
+def test_autotuners_keep_results(tmp_path, monkeypatch):
+    monkeypatch.setenv("OLBGFX_TUNE_CACHE", str(tmp_path / "tune.json"))
+    _, a = _members("f32", sizes=[10, 12])
+    _, b = _members("f32", sizes=[10, 12])
+    res = autotune.tune_lattice(a[0], steps=2, blocks=(64, 256), waves=(0, 3))
+    assert res["block"] in (64, 256) and len(res["trials"]) == 4
+    b[0].collide_and_stream(4 * (1 + 2))  # four trials of one untimed + two timed steps
+    np.testing.assert_array_equal(a[0].get_populations(), b[0].get_populations())
+    ens = Ensemble(a)
+    r2 = autotune.tune_ensemble(ens, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2))
+    assert len(r2["trials"]) == 4
+    cached = autotune.tune_lattice(a[0], steps=2)
+    assert cached.get("cached")
