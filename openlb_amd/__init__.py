@@ -297,7 +297,8 @@ class Lattice:
             "sumUSqr": out[2],
             "maxUSqr": out[3],
             "averageRho": out[1] / n if n else 0.0,
-            "averageEnergy": out[2] / n if n else 0.0,
+            # energy is 0.5 u^2 (core/latticeStatistics.hh:89)
+            "averageEnergy": 0.5 * out[2] / n if n else 0.0,
             "maxU": float(np.sqrt(out[3])),
         }
 

@@ -299,3 +299,20 @@ def ensemble_cavities(count: int = 64, nmin: int = 32, nmax: int = 128, umin: fl
         u = umin + (umax - umin) * rng.uniform()
         out.append(box_cavity(n, u_lid=u, re=re, padding=padding, dtype=dtype))
     return out
+
+
+def cavity3d(res: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,
+             dtype=np.float64) -> Case:
+    """The lid-driven cavity of examples/laminar/cavity3dBenchmark at
+    RESOLUTION res: (res+1)^3 nodes, wall shell (material 2), lid face y = res
+    without its rim (material 3), fluid inside (material 1)
+    (cavity3d.cpp:44-85), with BounceBack walls and a BounceBackVelocity lid
+    moving at u_lid; nu = u_lid * res / re (res = 64, u = 0.1, Re = 100 give
+    nu = 0.064, tau = 0.692)."""
+    c = box_cavity(res + 1, u_lid=u_lid, re=re, padding=padding, dtype=dtype)
+    nu = u_lid * res / re
+    omega = relaxation_frequency(nu)
+    c.name = f"cavity3d_res{res}"
+    c.dynamics[1] = Dynamics(BGK, omega=omega)
+    c.info.update({"nu": nu, "omega": omega, "res": res})
+    return c

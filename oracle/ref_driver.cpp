@@ -1,0 +1,426 @@
+/*
+ * ref_driver.cpp -- runs OpenLB's own CPU path on the workloads of this
+ * repository and writes binary dumps used as golden fixtures.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the OpenLB
+ * sources where they lie (never copied); the binary lands in oracle/_ref/ and
+ * is a checker: tests compare the gfx950 platform and the C restatement with
+ * what it writes.  It is our own driver code calling the public OpenLB API
+ * (SuperGeometry, SuperLattice, dynamics/boundary setters), the way an OpenLB
+ * case does.
+ *
+ * Usage (REF_VALUE = float or double decides the value type):
+ *   olb_ref cavity   RES STEPS ULID RE OUT        cavity3dBenchmark geometry, BGK +
+ *                                                BounceBack walls + BounceBackVelocity lid
+ *   olb_ref tgv      N STEPS U0 TAU OUT           periodic Taylor-Green, BGK
+ *   olb_ref porous   NX NY NZ SPHERES R TAU MAGIC FX STEPS OUT
+ *                                                periodic sphere pack, ForcedTRT + BounceBack
+ *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
+ *                                                convergence (the reference's KAT)
+ *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
+ *
+ * Dump format (little endian): text header line "OLBDUMP v1 <key=value ...>\n",
+ * then named arrays, each "ARRAY <name> <dtype> <count>\n" followed by raw data.
+ */
+#include <olb.h>
+
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+using namespace olb;
+using T = REF_VALUE;
+
+namespace {
+
+struct Dump {
+  std::ofstream out;
+  explicit Dump(const std::string& path, const std::string& header)
+    : out(path, std::ios::binary)
+  {
+    if (!out) {
+      throw std::runtime_error("cannot open " + path);
+    }
+    out << "OLBDUMP v1 " << header << "\n";
+  }
+  template <typename V>
+  void array(const std::string& name, const std::string& dtype, const std::vector<V>& v)
+  {
+    out << "ARRAY " << name << " " << dtype << " " << v.size() << "\n";
+    out.write(reinterpret_cast<const char*>(v.data()), sizeof(V) * v.size());
+  }
+};
+
+const char* tname() { return sizeof(T) == 4 ? "f4" : "f8"; }
+
+// materials of the padded block in CellID order
+std::vector<int32_t> materials(SuperGeometry<T,3>& G, int padding)
+{
+  auto& bg = G.getBlockGeometry(0);
+  std::vector<int32_t> m;
+  const int nx = bg.getNx(), ny = bg.getNy(), nz = bg.getNz();
+  for (int x = -padding; x < nx + padding; ++x) {
+    for (int y = -padding; y < ny + padding; ++y) {
+      for (int z = -padding; z < nz + padding; ++z) {
+        m.push_back(bg.get({x, y, z}));
+      }
+    }
+  }
+  return m;
+}
+
+// populations of the padded block in logical order (evaluation view)
+template <typename DESCRIPTOR>
+std::vector<T> populations(SuperLattice<T,DESCRIPTOR>& L)
+{
+  L.setProcessingContext(ProcessingContext::Evaluation);
+  auto& b = L.getBlock(0);
+  const std::size_t n = b.getNcells();
+  std::vector<T> f(DESCRIPTOR::q * n);
+  for (std::size_t iCell = 0; iCell < n; ++iCell) {
+    auto cell = b.get(iCell);
+    for (int i = 0; i < DESCRIPTOR::q; ++i) {
+      f[i * n + iCell] = cell[i];
+    }
+  }
+  return f;
+}
+
+template <typename DESCRIPTOR>
+std::string block_header(SuperLattice<T,DESCRIPTOR>& L)
+{
+  auto& b = L.getBlock(0);
+  return "nx=" + std::to_string(b.getNx()) + " ny=" + std::to_string(b.getNy()) +
+         " nz=" + std::to_string(b.getNz()) + " padding=" + std::to_string(b.getPadding()) +
+         " value=" + tname();
+}
+
+template <typename DESCRIPTOR>
+std::vector<int64_t> distances(SuperLattice<T,DESCRIPTOR>& L)
+{
+  std::vector<int64_t> d;
+  for (int i = 0; i < DESCRIPTOR::q; ++i) {
+    d.push_back(L.getBlock(0).getNeighborDistance(descriptors::c<DESCRIPTOR>(i)));
+  }
+  return d;
+}
+
+template <typename DESCRIPTOR>
+UnitConverter<T,DESCRIPTOR> lattice_units(T nu)
+{
+  // physical units equal lattice units: dx = dt = 1
+  return UnitConverter<T,DESCRIPTOR>(T(1), T(1), T(1), T(1), nu, T(1));
+}
+
+// ---- lid-driven cavity: the geometry of examples/laminar/cavity3dBenchmark ----
+int run_cavity(int res, int steps, double ulid, double re, const std::string& out)
+{
+  using DESCRIPTOR = descriptors::D3Q19<>;
+  const T length = 1;
+  const T dx = length / res;
+  Vector<T,3> origin((T)0);
+  IndicatorCuboid3D<T> meshCuboid(Vector<T,3>(length + T(0.5) * dx), origin);
+  CuboidDecomposition3D<T> cuboids(meshCuboid, dx, 1);
+  HeuristicLoadBalancer<T> balancer(cuboids);
+  SuperGeometry<T,3> G(cuboids, balancer, 3);
+
+  IndicatorCuboid3D<T> domain(Vector<T,3>(length + T(0.25) * dx), origin);
+  G.rename(0, 2, domain);
+  G.rename(2, 1, {1, 1, 1});
+  const T eps = dx;
+  IndicatorCuboid3D<T> lid(Vector<T,3>(length + 2 * eps, 2 * eps, length + 2 * eps),
+                           Vector<T,3>(-eps, length - eps, -eps));
+  G.rename(2, 3, 1, lid);
+  G.clean(false);
+  G.innerClean(false);
+  G.checkForErrors(false);
+
+  // Re = U N / nu with N the resolution (RES = 64 gives nu = 0.064 for U = 0.1)
+  const T nu = T(ulid * res / re);
+  SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), G);
+  L.setStatisticsOff();
+  dynamics::set<BGKdynamics>(L, G, 1);
+  boundary::set<boundary::BounceBack>(L, G, 2);
+  dynamics::set<BounceBackVelocity>(L, G, 3);
+  AnalyticalConst3D<T,T> uLid(T(ulid), T(0), T(0));
+  L.defineU(G.getMaterialIndicator(3), uLid);
+  const T omega = T(1) / (T(3) * nu + T(0.5));
+  L.setParameter<descriptors::OMEGA>(omega);
+  L.initialize();
+
+  Dump d(out, block_header(L) + " case=cavity res=" + std::to_string(res) + " steps=" +
+                  std::to_string(steps) + " omega=" + std::to_string((double)omega) +
+                  " ulid=" + std::to_string(ulid) + " nu=" + std::to_string((double)nu));
+  d.array("materials", "i4", materials(G, 3));
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), populations(L));
+  for (int t = 0; t < steps; ++t) {
+    L.collideAndStream();
+  }
+  d.array("f1", tname(), populations(L));
+  return 0;
+}
+
+// ---- Taylor-Green vortex ----
+struct TgvField : public AnalyticalF3D<T,T> {
+  double u0, k;
+  bool density;
+  TgvField(double u0_, int n, bool density_)
+    : AnalyticalF3D<T,T>(density_ ? 1 : 3), u0(u0_), k(2 * M_PI / n), density(density_) { }
+  bool operator()(T output[], const T input[]) override
+  {
+    const double x = k * input[0], y = k * input[1], z = k * input[2];
+    if (density) {
+      const double p = u0 * u0 / 16.0 * (std::cos(2 * x) + std::cos(2 * y)) * (std::cos(2 * z) + 2.0);
+      output[0] = T(1.0 + 3.0 * p);
+    } else {
+      output[0] = T(u0 * std::sin(x) * std::cos(y) * std::cos(z));
+      output[1] = T(-u0 * std::cos(x) * std::sin(y) * std::cos(z));
+      output[2] = T(0);
+    }
+    return true;
+  }
+};
+
+template <typename DESCRIPTOR>
+void build_periodic_box(int nx, int ny, int nz, CuboidDecomposition3D<T>*& cuboids,
+                        HeuristicLoadBalancer<T>*& balancer, SuperGeometry<T,3>*& G)
+{
+  cuboids = new CuboidDecomposition3D<T>(Vector<T,3>(T(0)), T(1), Vector<int,3>(nx, ny, nz), 1);
+  cuboids->setPeriodicity({true, true, true});
+  balancer = new HeuristicLoadBalancer<T>(*cuboids);
+  G = new SuperGeometry<T,3>(*cuboids, *balancer, 3);
+  G->rename(0, 1);
+  G->communicate();
+}
+
+int run_tgv(int n, int steps, double u0, double tau, const std::string& out, bool bench)
+{
+  using DESCRIPTOR = descriptors::D3Q19<>;
+  CuboidDecomposition3D<T>* cuboids;
+  HeuristicLoadBalancer<T>* balancer;
+  SuperGeometry<T,3>* G;
+  build_periodic_box<DESCRIPTOR>(n, n, n, cuboids, balancer, G);
+  const T nu = T((tau - 0.5) / 3.0);
+  SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), *G);
+  L.setStatisticsOff();
+  dynamics::set<BGKdynamics>(L, *G, 1);
+  L.setParameter<descriptors::OMEGA>(T(1.0 / tau));
+  TgvField rho(u0, n, true), u(u0, n, false);
+  L.iniEquilibrium(*G, 1, rho, u);
+  L.initialize();
+  if (bench) {
+    for (int t = 0; t < 2; ++t) {
+      L.collideAndStream();
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < steps; ++t) {
+      L.collideAndStream();
+    }
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("MLUPS %.3f\nseconds %.3f\n", double(n) * n * n * steps / s / 1e6, s);
+    return 0;
+  }
+  Dump d(out, block_header(L) + " case=tgv n=" + std::to_string(n) + " steps=" + std::to_string(steps) +
+                  " u0=" + std::to_string(u0) + " tau=" + std::to_string(tau));
+  d.array("materials", "i4", materials(*G, 3));
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), populations(L));
+  for (int t = 0; t < steps; ++t) {
+    L.collideAndStream();
+  }
+  d.array("f1", tname(), populations(L));
+  return 0;
+}
+
+// ---- periodic sphere pack, ForcedTRT + BounceBack ----
+int run_porous(int nx, int ny, int nz, const std::string& spheres, double radius, double tau,
+               double magic, double fx, int steps, const std::string& out)
+{
+  using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
+  CuboidDecomposition3D<T>* cuboids;
+  HeuristicLoadBalancer<T>* balancer;
+  SuperGeometry<T,3>* G;
+  build_periodic_box<DESCRIPTOR>(nx, ny, nz, cuboids, balancer, G);
+  std::ifstream in(spheres);
+  double cx, cy, cz;
+  const int ext[3] = {nx, ny, nz};
+  while (in >> cx >> cy >> cz) {
+    // the sphere and its periodic images
+    for (int ix = -1; ix <= 1; ++ix) {
+      for (int iy = -1; iy <= 1; ++iy) {
+        for (int iz = -1; iz <= 1; ++iz) {
+          Vector<T,3> c(T(cx + ix * ext[0]), T(cy + iy * ext[1]), T(cz + iz * ext[2]));
+          IndicatorSphere3D<T> s(c, T(radius));
+          G->rename(1, 2, s);
+        }
+      }
+    }
+  }
+  G->communicate();
+  const T nu = T((tau - 0.5) / 3.0);
+  SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), *G);
+  L.setStatisticsOff();
+  dynamics::set<ForcedTRTdynamics>(L, *G, 1);
+  dynamics::set<BounceBack>(L, *G, 2);
+  L.setParameter<descriptors::OMEGA>(T(1.0 / tau));
+  L.setParameter<collision::TRT::MAGIC>(T(magic));
+  AnalyticalConst3D<T,T> force(T(fx), T(0), T(0));
+  L.defineField<descriptors::FORCE>(*G, 1, force);
+  L.initialize();
+  Dump d(out, block_header(L) + " case=porous steps=" + std::to_string(steps) + " tau=" +
+                  std::to_string(tau) + " magic=" + std::to_string(magic) + " fx=" + std::to_string(fx));
+  d.array("materials", "i4", materials(*G, 3));
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), populations(L));
+  for (int t = 0; t < steps; ++t) {
+    L.collideAndStream();
+  }
+  d.array("f1", tname(), populations(L));
+  return 0;
+}
+
+
+// ---- Poiseuille pipe, forced BGK + BounceBack, run to convergence: the set-up
+// of the reference's known-answer test tests/benchmarks/testPoiseuille3DBGK.cpp
+// (forced, bounceBack row), driven through the same public API ----
+int run_poiseuille(int n, const std::string& out)
+{
+  using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
+  const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
+  UnitConverterFromResolutionAndRelaxationTime<T,DESCRIPTOR> const converter(n, T(0.8), T(1), T(1), T(1) / Re, T(1));
+  const T dx = converter.getPhysDeltaX();
+  Vector<T,3> a0(0, r, r), a1(lx, r, r);
+  IndicatorCylinder3D<T> pipe(a0, a1, r);
+  IndicatorLayer3D<T> extended(pipe, dx);
+  CuboidDecomposition3D<T> cuboids(extended, dx, 1);
+  cuboids.setPeriodicity({true, false, false});
+  HeuristicLoadBalancer<T> balancer(cuboids);
+  SuperGeometry<T,3> G(cuboids, balancer, 2);
+
+  Vector<T,3> c0(-dx * T(0.2) - 3 * dx, r, r), c1(lx + 3 * dx, r, r);
+  IndicatorCylinder3D<T> pipeLong(c0, c1, r);
+  G.rename(0, 2);
+  G.rename(2, 1, pipeLong);
+  G.clean(false);
+  G.innerClean(false);
+  G.checkForErrors(false);
+
+  SuperLattice<T,DESCRIPTOR> L(converter, G);
+  L.template defineDynamics<NoDynamics>(G, 0);
+  L.template defineDynamics<ForcedBGKdynamics>(G, 1);
+  L.template defineDynamics<BounceBack>(G, 2);
+  const T D = converter.getLatticeLength(diameter);
+  std::vector<T> f(3, T());
+  f[0] = 4. * converter.getLatticeViscosity() * converter.getCharLatticeVelocity() / (D * D / 4.);
+  AnalyticalConst3D<T,T> force(f);
+  L.template defineField<descriptors::FORCE>(G, 1, force);
+  L.template defineField<descriptors::FORCE>(G, 2, force);
+  std::vector<T> axisPoint = {lx, r, r};
+  std::vector<T> axisDirection = {1, 0, 0};
+  CirclePoiseuille3D<T> poiseuilleU(axisPoint, axisDirection, converter.getCharLatticeVelocity(), r);
+  AnalyticalConst3D<T,T> rhoF(1);
+  L.defineRhoU(G, 1, rhoF, poiseuilleU);
+  L.iniEquilibrium(G, 1, rhoF, poiseuilleU);
+  L.defineRhoU(G, 2, rhoF, poiseuilleU);
+  L.iniEquilibrium(G, 2, rhoF, poiseuilleU);
+  const T omega = converter.getLatticeRelaxationFrequency();
+  L.template setParameter<descriptors::OMEGA>(omega);
+  L.initialize();
+
+  const int padding = G.getOverlap();
+  std::vector<T> f0 = populations(L);
+  util::ValueTracer<T> converge(converter.getLatticeTime(physInterval), residuum);
+  std::vector<double> energy;
+  int iT;
+  for (iT = 0; iT < (int)converter.getLatticeTime(maxPhysT); ++iT) {
+    if (converge.hasConverged()) {
+      break;
+    }
+    L.collideAndStream();
+    energy.push_back(L.getStatistics().getAverageEnergy());
+    converge.takeValue(L.getStatistics().getAverageEnergy(), false);
+  }
+
+  // relative L2 velocity error over the fluid (material 1) as the test computes it
+  int tmp[] = {int()};
+  T result[2] = {T(), T()}, resultSol[2] = {T(), T()};
+  CirclePoiseuille3D<T> uSol(axisPoint, axisDirection, converter.getCharPhysVelocity(), r);
+  SuperLatticePhysVelocity3D<T,DESCRIPTOR> u(L, converter);
+  SuperLatticeFfromAnalyticalF3D<T,DESCRIPTOR> uSolLattice(uSol, L);
+  SuperL2Norm3D<T> uL2Norm(uSolLattice - u, G, 1);
+  SuperL2Norm3D<T> uSolL2Norm(uSolLattice, G, 1);
+  uL2Norm(result, tmp);
+  uSolL2Norm(resultSol, tmp);
+  const double velocityErrorL2 = result[0] / resultSol[0];
+
+  // analytic velocity at every padded cell (zero outside material 1)
+  auto& bg = G.getBlockGeometry(0);
+  std::vector<T> usol;
+  for (int x = -padding; x < bg.getNx() + padding; ++x) {
+    for (int y = -padding; y < bg.getNy() + padding; ++y) {
+      for (int z = -padding; z < bg.getNz() + padding; ++z) {
+        T v[3] = {0, 0, 0};
+        if (bg.get({x, y, z}) == 1) {
+          Vector<T,3> physR;
+          bg.getPhysR(physR, {x, y, z});
+          uSol(v, physR.data());
+        }
+        usol.insert(usol.end(), v, v + 3);
+      }
+    }
+  }
+  char buf[512];
+  std::snprintf(buf, sizeof(buf),
+                " case=poiseuille n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g deltaT=%zu "
+                "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu",
+                n, iT, (double)omega, (double)f[0], (double)converter.getConversionFactorVelocity(),
+                converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2,
+                converter.getLatticeTime(maxPhysT));
+  Dump d(out, block_header(L) + buf);
+  d.array("materials", "i4", materials(G, padding));
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), f0);
+  d.array("f1", tname(), populations(L));
+  d.array("energy", "f8", energy);
+  d.array("usol", tname(), usol);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  olb::initialize(&argc, &argv, false, false);
+  if (argc < 2) {
+    std::cerr << "usage: see header of ref_driver.cpp\n";
+    return 2;
+  }
+  const std::string mode = argv[1];
+  try {
+    if (mode == "cavity" && argc == 7) {
+      return run_cavity(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6]);
+    }
+    if (mode == "tgv" && argc == 7) {
+      return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6], false);
+    }
+    if (mode == "bench_tgv" && argc == 4) {
+      return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), 0.04, 0.56, "", true);
+    }
+    if (mode == "poiseuille" && argc == 4) {
+      return run_poiseuille(std::stoi(argv[2]), argv[3]);
+    }
+    if (mode == "porous" && argc == 12) {
+      return run_porous(std::stoi(argv[2]), std::stoi(argv[3]), std::stoi(argv[4]), argv[5], std::stod(argv[6]),
+                        std::stod(argv[7]), std::stod(argv[8]), std::stod(argv[9]), std::stoi(argv[10]), argv[11]);
+    }
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  }
+  std::cerr << "bad arguments\n";
+  return 2;
+}

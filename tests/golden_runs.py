@@ -1,0 +1,132 @@
+"""Set-up of runs from the golden fixtures of tests/golden/ (written by the
+reference's own CPU path, see tests/golden/make_fixtures.py)."""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+
+from openlb_amd import BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FORCED_BGK, FORCED_TRT, NO_DYNAMICS, Dynamics
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name: str):
+    d = np.load(os.path.join(GOLDEN, name + ".npz"))
+    meta = json.loads(str(d["meta"]))
+    return meta, d
+
+
+def _as_value(x: float, storage: str) -> float:
+    """a parameter as the reference's value type holds it"""
+    return float(np.float32(x)) if storage != "f64" else float(x)
+
+
+def dynamics_for(meta: dict, storage: str) -> dict:
+    """material -> Dynamics of the fixture's case, parameters rounded to the
+    value type the way the reference driver forms them"""
+    case = meta["case"]
+    if case == "cavity":
+        if storage == "f64":
+            nu = 0.1 * meta["res"] / 100.0
+            omega = 1.0 / (3.0 * nu + 0.5)
+        else:
+            f = np.float32
+            nu = f(0.1 * meta["res"] / 100.0)
+            omega = float(f(1) / (f(3) * nu + f(0.5)))
+        return {1: Dynamics(BGK, omega=omega), 2: Dynamics(BOUNCE_BACK),
+                3: Dynamics(BOUNCE_BACK_VELOCITY, wall_velocity=(_as_value(meta["ulid"], storage), 0, 0),
+                            wall_rho=1.0)}
+    if case == "tgv":
+        return {1: Dynamics(BGK, omega=_as_value(1.0 / meta["tau"], storage))}
+    if case == "porous":
+        return {1: Dynamics(FORCED_TRT, omega=_as_value(1.0 / meta["tau"], storage),
+                            magic=_as_value(meta["magic"], storage),
+                            force=(_as_value(meta["fx"], storage), 0.0, 0.0)),
+                2: Dynamics(BOUNCE_BACK)}
+    if case == "poiseuille":
+        return {1: Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0)),
+                2: Dynamics(BOUNCE_BACK)}
+    raise ValueError(case)
+
+
+def periodic_for(meta: dict):
+    return {"cavity": (False,) * 3, "tgv": (True,) * 3, "porous": (True,) * 3,
+            "poiseuille": (True, False, False)}[meta["case"]]
+
+
+def build(lattice_cls, meta, d, storage: str):
+    """a lattice with the fixture's geometry, dynamics and t = 0 populations"""
+    ext = (meta["nx"], meta["ny"], meta["nz"])
+    lat = lattice_cls(ext, meta["padding"], periodic_for(meta), storage)
+    mats = d["materials"].astype(np.uint8)
+    dyns = dynamics_for(meta, storage)
+    lat.set_dynamics(0, Dynamics(NO_DYNAMICS))
+    lut = np.zeros(256, dtype=np.uint8)
+    for slot, (mat, dyn) in enumerate(sorted(dyns.items()), start=1):
+        lat.set_dynamics(slot, dyn)
+        lut[mat] = slot
+    lat.set_dynamics_mask(lut[mats])
+    f = np.zeros((19, lat.ncells))
+    f[:, d["cells"]] = d["f0"]
+    lat.set_populations(f)
+    return lat
+
+
+def compared_cells(meta, d) -> np.ndarray:
+    """columns of f1 that are compared: the fluid for the closed cavity (its
+    walls also hold values streamed in from the padding, which the reference
+    keeps in rings of a different length), every core cell otherwise"""
+    if meta["case"] == "cavity":
+        return np.flatnonzero(d["materials"][d["cells"]] == 1)
+    return np.arange(d["cells"].size)
+
+
+def value_tracer_run(lat, deltaT: int, residuum: float, max_steps: int):
+    """util::ValueTracer convergence loop of the reference's Poiseuille test
+    (utilities/benchmarkUtil.hh:78-128, testPoiseuille3DBGK.cpp:353-371)"""
+    values, energy = [], []
+    lat.set_statistics(True)
+    it = 0
+    for it in range(max_steps):
+        if len(values) >= deltaT:
+            avg = sum(values) / len(values)
+            std = np.sqrt(sum((v - avg) ** 2 for v in values) / (len(values) - 1))
+            if abs(std / avg) < residuum:
+                break
+        lat.collide_and_stream(1)
+        e = lat.get_statistics()["averageEnergy"]
+        energy.append(e)
+        values.append(e)
+        if len(values) > deltaT:
+            values.pop(0)
+    else:
+        it = max_steps
+    return it, np.array(energy)
+
+
+def core_fluid(meta, d) -> np.ndarray:
+    """positions (into d["fluid"]) of the fluid cells inside the core; the
+    reference keeps periodic copies of them in the padding"""
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    xyz = np.array(np.unravel_index(d["fluid"], shape)).T - p
+    inside = np.all((xyz >= 0) & (xyz < np.array([meta["nx"], meta["ny"], meta["nz"]])), axis=1)
+    return np.flatnonzero(inside)
+
+
+def velocity_error_l2(lat, meta, d) -> float:
+    """relative L2 error of the physical velocity over the fluid, as the test
+    computes it (testPoiseuille3DBGK.cpp:262-275): u includes the F/2 shift
+    of the forced momenta"""
+    from tests.helpers import C
+
+    sel = core_fluid(meta, d)
+    f = lat.get_populations()[:, d["fluid"][sel]].astype(np.float64)
+    rho = f.sum(0) + 1.0
+    u = (C.T @ f) / rho
+    u[0] += 0.5 * meta["force"]
+    u_phys = u.T * meta["u_factor"]
+    sol = d["usol_fluid"][sel]
+    return float(np.sqrt(np.sum((sol - u_phys) ** 2) / np.sum(sol ** 2)))

@@ -1,0 +1,73 @@
+"""The oracle and the case builders against the reference's own CPU path
+(fixtures in tests/golden/, written by OpenLB through oracle/_ref) and the
+reference's known-answer test (Poiseuille, forced BGK + BounceBack)."""
+import numpy as np
+import pytest
+
+from openlb_amd import cases
+from oracle.oracle import OracleLattice
+from tests.golden_runs import build, compared_cells, core_fluid, load, value_tracer_run, velocity_error_l2
+from tests.helpers import rel_err_unshifted
+
+FIXTURES = ["cavity3d_res12", "tgv12", "porous16x12x12"]
+TOL = {"f32": 1e-5, "f64": 1e-12}
+
+
+def test_cavity_geometry_bit_exact():
+    """cavity3dBenchmark's materials (cavity3d.cpp:58-85) at RESOLUTION 12"""
+    meta, d = load("cavity3d_res12_f32")
+    c = cases.cavity3d(12)
+    np.testing.assert_array_equal(c.materials.reshape(-1), d["materials"])
+    assert c.extent == (meta["nx"], meta["ny"], meta["nz"])
+
+
+def test_porous_geometry_bit_exact_in_core():
+    meta, d = load("porous16x12x12_f32")
+    m = cases.porous_materials(tuple(meta["extent"]), np.array(meta["spheres"]), meta["radius"], 3)
+    p = 3
+    ref = d["materials"].reshape(m.shape)
+    np.testing.assert_array_equal(m[p:-p, p:-p, p:-p], ref[p:-p, p:-p, p:-p])
+
+
+def test_tgv_geometry_in_core():
+    meta, d = load("tgv12_f32")
+    c = cases.tgv(12)
+    ref = d["materials"].reshape(c.shape)
+    p = 3
+    np.testing.assert_array_equal(c.materials[p:-p, p:-p, p:-p], ref[p:-p, p:-p, p:-p])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_neighbor_distances_bit_exact(name):
+    meta, d = load(name + "_f32")
+    lat = OracleLattice((meta["nx"], meta["ny"], meta["nz"]), meta["padding"])
+    np.testing.assert_array_equal(lat.neighbor_distances(), d["distances"])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_matches_reference(name, storage):
+    meta, d = load(f"{name}_{storage}")
+    lat = build(OracleLattice, meta, d, storage)
+    lat.collide_and_stream(meta["steps"])
+    cols = compared_cells(meta, d)
+    got = lat.get_populations()[:, d["cells"]][:, cols]
+    err = rel_err_unshifted(got, d["f1"][:, cols])
+    print(name, storage, "max rel err vs reference", err)
+    assert err <= TOL[storage]
+
+
+def test_oracle_poiseuille_known_answer():
+    """testPoiseuille3DBGK.cpp:55-56 (forced, bounceBack, N = 11): converges
+    at iT = 422 with velocity L2 error 0.158042 (+- 1e-6)"""
+    meta, d = load("poiseuille11_forced_bounceback_f64")
+    lat = build(OracleLattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 422 == meta["iT"]
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    assert abs(err - 0.158042) <= 1e-6
+    assert abs(err - meta["velocityErrorL2"]) <= 1e-9
+    sel = core_fluid(meta, d)
+    f1 = lat.get_populations()[:, d["fluid"][sel]]
+    assert rel_err_unshifted(f1, d["f1_fluid"][:, sel]) <= 1e-12

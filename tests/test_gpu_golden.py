@@ -1,0 +1,61 @@
+"""The gfx950 kernels against the reference's own CPU path (fixtures written
+by OpenLB through oracle/_ref) and against the reference's known-answer test.
+Tolerances: FP32 populations within 1e-5 relative (unshifted), FP64 within
+1e-12; the known-answer test to its own 1e-6 and the exact iteration count."""
+import numpy as np
+import pytest
+
+from openlb_amd import Lattice
+from tests.golden_runs import build, compared_cells, core_fluid, load, value_tracer_run, velocity_error_l2
+from tests.helpers import rel_err_unshifted
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["cavity3d_res12", "tgv12", "porous16x12x12"]
+TOL = {"f32": 1e-5, "f64": 1e-12}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gpu_matches_reference(name, storage):
+    print(f"[golden] {name} {storage}", flush=True)
+    meta, d = load(f"{name}_{storage}")
+    lat = build(Lattice, meta, d, storage)
+    np.testing.assert_array_equal(lat.neighbor_distances(), d["distances"])
+    lat.collide_and_stream(meta["steps"])
+    cols = compared_cells(meta, d)
+    got = lat.get_populations()[:, d["cells"]][:, cols]
+    err = rel_err_unshifted(got, d["f1"][:, cols])
+    print(name, storage, "max rel err vs reference", err)
+    assert err <= TOL[storage]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gpu_fp16_storage_near_reference(name):
+    """FP16 shifted storage from the same start: velocity within 1e-2 relative L2
+    of the reference's FP64 run"""
+    from tests.helpers import rel_l2, velocity
+    meta, d = load(f"{name}_f64")
+    lat = build(Lattice, meta, d, "f16")
+    lat.collide_and_stream(meta["steps"])
+    cols = compared_cells(meta, d)
+    ug = velocity(lat.get_populations()[:, d["cells"]][:, cols])
+    ur = velocity(d["f1"][:, cols])
+    err = rel_l2(ug, ur)
+    print(name, "fp16 velocity rel L2 vs reference fp64", err)
+    assert err <= 1e-2
+
+
+def test_gpu_poiseuille_known_answer():
+    """testPoiseuille3DBGK.cpp:55-56 (forced, bounceBack, N = 11) on the GPU in
+    FP64: iteration count 422 and velocity L2 error 0.158042 +- 1e-6"""
+    meta, d = load("poiseuille11_forced_bounceback_f64")
+    lat = build(Lattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 422
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    print("poiseuille gpu velocity error", err)
+    assert abs(err - 0.158042) <= 1e-6
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
