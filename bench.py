@@ -15,6 +15,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -41,6 +42,11 @@ def parse_args():
     ap.add_argument("--ens-storages", default="f32,f16")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--acc-size", type=int, default=512)
+    ap.add_argument("--acc-steps", type=int, default=1000)
+    ap.add_argument("--no-porous", action="store_true")
+    ap.add_argument("--porous-steps", type=int, default=500)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     return ap.parse_args()
@@ -146,6 +152,76 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     return out
 
 
+def accuracy(torch, size, steps):
+    """config 3: TGV at size^3 run `steps` steps in FP64, FP32, FP16 and BF16
+    storage from the same analytic start; velocity error of each against the
+    FP64 run (relative L2 over the core, max abs relative to u0), and the FP64
+    kinetic-energy decay against the analytic rate"""
+    from openlb_amd import Lattice, cases
+
+    n = size
+    core = None
+    u_ref = None
+    out = {"size": n, "steps": steps}
+    for st in ("f64", "f32", "f16", "bf16"):
+        case = cases.tgv(n, u0=0.04, tau=0.56, dtype=np.float64 if st == "f64" else np.float32)
+        if core is None:
+            core = case.core_mask().reshape(-1)
+        lat = Lattice(case.extent, case.padding, case.periodic, st)
+        case.apply(lat)
+        del case
+        e0 = lat.reduce_moments()["sumUSqr"]
+        t0 = time.perf_counter()
+        lat.collide_and_stream(steps)
+        lat.synchronize()
+        dt = time.perf_counter() - t0
+        m = lat.reduce_moments()
+        _, u = lat.compute_rho_u()
+        lat.close()
+        u = u[core].astype(np.float64)
+        if st == "f64":
+            u_ref = u
+            nu = (0.56 - 0.5) / 3.0
+            rate = -np.log(m["sumUSqr"] / e0) / steps
+            out["f64"] = {"energy_decay_rate": rate,
+                          "analytic_rate": 2 * nu * 3 * (2 * np.pi / n) ** 2,
+                          "seconds": dt}
+            continue
+        diff = u - u_ref
+        out[st] = {
+            "velocity_l2_vs_f64": float(np.sqrt(np.sum(diff ** 2) / np.sum(u_ref ** 2))),
+            "velocity_max_vs_f64_over_u0": float(np.max(np.abs(diff)) / 0.04),
+            "mass_drift_per_cell": float(m["mass"] / n ** 3),
+            "seconds": dt,
+        }
+        del u, diff
+    return out
+
+
+def time_porous(torch, steps, storages=("f32", "f16")):
+    """config 4: 384x128x128 periodic sphere pack (r = 6, porosity ~0.7),
+    ForcedTRT (Lambda = 3/16) with g_x = 1e-6, BounceBack solids"""
+    from openlb_amd import Lattice, cases
+
+    res = {}
+    case = cases.porous_channel((384, 128, 128), radius=6.0, porosity=0.7, seed=20261202, force=1e-6,
+                                dtype=np.float32)
+    handle = torch.cuda.current_stream().cuda_stream
+    cells = int(np.prod(case.extent))
+    for st in storages:
+        lat = Lattice(case.extent, case.padding, case.periodic, st)
+        case.apply(lat)
+        lat.collide_and_stream(10, handle)
+        ms = lat.time_steps(steps, handle)
+        m = lat.reduce_moments(handle)
+        res[st] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms_per_step": ms / steps,
+                   "mean_ux": m["momentum"][0] / m["nCells"]}
+        lat.close()
+    res["porosity"] = case.info["porosity"]
+    res["spheres"] = int(len(case.info["spheres"]))
+    return res
+
+
 def copy_bandwidth(torch, nbytes=4 << 30, reps=10):
     """measured device copy bandwidth (read + write bytes / time)"""
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -173,8 +249,9 @@ def cpu_baseline(size, steps):
     if os.path.exists(ref):
         env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
         try:
-            out = subprocess.run([ref, "bench_tgv", str(size), str(steps)], env=env, capture_output=True,
-                                 text=True, timeout=300, check=True).stdout
+            with tempfile.TemporaryDirectory() as scratch:  # the reference writes ./tmp/
+                out = subprocess.run([ref, "bench_tgv", str(size), str(steps)], env=env, cwd=scratch,
+                                     capture_output=True, text=True, timeout=300, check=True).stdout
             mlups = float([ln for ln in out.splitlines() if ln.startswith("MLUPS")][-1].split()[1])
             return {"value": mlups / 1000.0, "unit": "GLUPS", "cores": cores, "kind": "reference",
                     "sample": f"OpenLB CPU_SIMD+OpenMP BGK TGV {size}^3 FP32, {steps} steps after 2 warm-up"}
@@ -235,6 +312,14 @@ def main():
             print(f"# ensemble {st}: graph {ens[st]['graph_steps_per_s']:.1f} steps/s "
                   f"({ens[st]['graph_glups']:.2f} GLUPS), separate {ens[st]['separate_steps_per_s']:.1f} "
                   f"steps/s, x{ens[st]['speedup_graph_vs_separate']:.2f}", file=sys.stderr)
+    porous = None if args.no_porous else time_porous(torch, args.porous_steps)
+    if porous:
+        print(f"# porous: f32 {porous['f32']['glups']:.2f} GLUPS, f16 {porous['f16']['glups']:.2f} GLUPS",
+              file=sys.stderr)
+    acc = None if args.no_accuracy else accuracy(torch, args.acc_size, args.acc_steps)
+    if acc:
+        print("# accuracy vs f64: " + ", ".join(f"{k} L2 {acc[k]['velocity_l2_vs_f64']:.2e}"
+                                                for k in ("f32", "f16", "bf16")), file=sys.stderr)
     copy_gbps = copy_bandwidth(torch)
     head = res.get("f32") or next(iter(res.values()))
     head_storage = "f32" if "f32" in res else storages[0]
@@ -277,6 +362,8 @@ def main():
         "fp16_over_fp32": round(res["f16"]["glups"] / res["f32"]["glups"], 3) if "f16" in res and "f32" in res else None,
         "ensemble": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in ens.items()},
+        "porous": porous,
+        "accuracy": acc,
         "roofline": roofline,
         "cpu_baseline": base,
     }
