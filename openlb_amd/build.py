@@ -54,6 +54,16 @@ def build_library() -> str:
     return LIB
 
 
+def build_host_examples() -> None:
+    """C++ cases written against the host mirror (openlb_amd/host); plain g++
+    over the C ABI, no HIP headers"""
+    src = os.path.join(PKG, "host", "examples", "cavity3d.cpp")
+    out = os.path.join(PKG, "host", "examples", "cavity3d")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(PKG, "host"), src, "-L", PKG, "-lolbgfx",
+          "-Wl,-rpath,$ORIGIN/../..", "-o", out])
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """Compile the CPU restatement (and the reference checker if possible)."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
@@ -63,6 +73,7 @@ def build_oracle(with_reference: bool = True) -> None:
 
 def build_all() -> None:
     build_library()
+    build_host_examples()
     build_oracle()
 
 

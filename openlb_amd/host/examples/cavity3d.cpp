@@ -1,0 +1,126 @@
+/*
+ * Lid-driven cavity on the gfx950 platform, written the way an OpenLB case
+ * is (examples/laminar/cavity3dBenchmark/cavity3d.cpp with
+ * LID_DRIVEN_BOUNCE_BACK): geometry, lattice, initial values, then timed
+ * collideAndStream steps after 10 warm-up steps, printing
+ * "RESOLUTION, TIME_STEPS, CUBOIDS, ranks, threads, MLUPs" as the reference's
+ * benchmark does (cavity3d.cpp:225-234).
+ *
+ *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
+ *            [--STORAGE f32|f16|bf16] [--DUMP file]
+ *
+ * --DUMP writes the materials and the populations of the padded block after
+ * the run (logical order, float32) for the tests.
+ */
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+
+#include "olb_gfx950.h"
+
+using namespace olb;
+using T = float;
+using DESCRIPTOR = descriptors::D3Q19;
+
+struct Parameters {
+  int resolution = 64;
+  int timeSteps = 100;
+  T lidVelocity = T(0.1);
+  T reynolds = T(100);
+  Storage storage = Storage::F32;
+  std::string dump;
+};
+
+static Parameters fromCLI(int argc, char** argv)
+{
+  Parameters p;
+  for (int i = 1; i + 1 < argc; i += 2) {
+    const std::string k = argv[i], v = argv[i + 1];
+    if (k == "--RESOLUTION") p.resolution = std::stoi(v);
+    else if (k == "--TIME_STEPS") p.timeSteps = std::stoi(v);
+    else if (k == "--LID_VELOCITY") p.lidVelocity = std::stof(v);
+    else if (k == "--REYNOLDS") p.reynolds = std::stof(v);
+    else if (k == "--DUMP") p.dump = v;
+    else if (k == "--STORAGE") {
+      p.storage = v == "f16" ? Storage::F16 : (v == "bf16" ? Storage::BF16 : Storage::F32);
+    } else {
+      throw std::invalid_argument("unknown option " + k);
+    }
+  }
+  return p;
+}
+
+// the cavity3dBenchmark geometry at RESOLUTION n: (n+1)^3 nodes, walls
+// (material 2), the lid face y = n without its rim (material 3), fluid (1)
+static void prepareGeometry(SuperGeometry<T>& geometry, int n)
+{
+  geometry.rename(0, 2);
+  geometry.rename(2, 1, {1, 1, 1});
+  geometry.rename(2, 3, [n](int x, int y, int z) {
+    return y == n && x > 0 && x < n && z > 0 && z < n;
+  });
+}
+
+static void prepareLattice(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry,
+                           const Parameters& p)
+{
+  dynamics::set<BGKdynamics>(lattice, geometry, 1);
+  boundary::set<boundary::BounceBack>(lattice, geometry, 2);
+  dynamics::set<BounceBackVelocity>(lattice, geometry, 3);
+  lattice.setStatisticsOff();
+}
+
+static void setInitialValues(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry,
+                             const Parameters& p)
+{
+  lattice.defineU(geometry, 3, {p.lidVelocity, T(0), T(0)});
+  const T nu = p.lidVelocity * T(p.resolution) / p.reynolds;
+  lattice.setParameter<descriptors::OMEGA>(T(1) / (T(3) * nu + T(0.5)));
+  lattice.initialize();
+}
+
+int main(int argc, char** argv)
+{
+  try {
+    const Parameters p = fromCLI(argc, argv);
+    const int n = p.resolution;
+    SuperGeometry<T> geometry({n + 1, n + 1, n + 1}, 3);
+    prepareGeometry(geometry, n);
+    SuperLattice<T, DESCRIPTOR> lattice(geometry, p.storage);
+    prepareLattice(lattice, geometry, p);
+    setInitialValues(lattice, geometry, p);
+
+    for (int iT = 0; iT < 10; ++iT) {
+      lattice.collideAndStream();
+    }
+    check(olbgfx_synchronize(lattice.handle()));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int iT = 0; iT < p.timeSteps; ++iT) {
+      lattice.collideAndStream();
+    }
+    check(olbgfx_synchronize(lattice.handle()));
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double mlups = double(geometry.getNvoxel()) * p.timeSteps / s / 1e6;
+
+    lattice.setProcessingContext(ProcessingContext::Evaluation);
+    std::cout << p.resolution << ", " << p.timeSteps << ", 1, 1, 1, " << mlups << std::endl;
+    if (!p.dump.empty()) {
+      std::ofstream out(p.dump, std::ios::binary);
+      const auto& m = geometry.materials();
+      out.write(reinterpret_cast<const char*>(m.data()), m.size());
+      for (int i = 0; i < DESCRIPTOR::q; ++i) {
+        for (CellID c = 0; c < (CellID)lattice.getNcells(); ++c) {
+          const T f = lattice.population(c, i);
+          out.write(reinterpret_cast<const char*>(&f), sizeof(T));
+        }
+      }
+    }
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

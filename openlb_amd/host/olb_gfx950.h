@@ -1,0 +1,568 @@
+/*
+ * olb_gfx950.h -- C++ host side of the gfx950 platform, mirroring the OpenLB
+ * calls an application makes on the D3Q19 hot path:
+ *
+ *   SuperGeometry::rename                      geometry/superGeometry.h, blockGeometry.hh:401-466
+ *   SuperLattice::defineDynamics<DYNAMICS>     core/superLattice.h:228-250
+ *   dynamics::set / boundary::set              case/setters.h:303-345, boundary/setBoundary.h:150-157
+ *   SuperLattice::setParameter<OMEGA / MAGIC>  core/superLattice.h (setParameter)
+ *   SuperLattice::defineField<FORCE>, defineU  core/superLattice.h:255-330
+ *   SuperLattice::iniEquilibrium               core/blockLattice.hh:202-236
+ *   SuperLattice::initialize, collideAndStream core/superLattice.hh:248-266, 663-725
+ *   setStatisticsOn/Off, getStatistics         core/superLattice.h, core/latticeStatistics.h:43-127
+ *   setProcessingContext(Evaluation)           core/superLattice.hh:184-195
+ *
+ * Everything below calls the C ABI of include/olbgfx.h; no HIP header is
+ * included, so a host translation unit builds with any C++17 compiler and
+ * links libolbgfx.so.  The dynamics are the closed set the platform provides;
+ * asking for another one fails at compile time (there is no tag for it), and
+ * an unknown kind passed through the ABI fails with std::domain_error, as the
+ * reference's platform dispatch does for unsupported operators.
+ *
+ * One SuperLattice is one block (one cuboid) on one GPU, in lattice units.
+ */
+#ifndef OLB_GFX950_H
+#define OLB_GFX950_H
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "olbgfx.h"
+
+namespace olb {
+
+using CellID = std::uint32_t;
+
+/// error reporting as the reference does it: exceptions, by kind
+inline void check(olbgfx_status status)
+{
+  if (status == OLBGFX_OK) {
+    return;
+  }
+  const std::string msg = olbgfx_last_error();
+  switch (status) {
+  case OLBGFX_ERR_INVALID:
+    throw std::invalid_argument(msg);
+  case OLBGFX_ERR_UNSUPPORTED:
+    throw std::domain_error(msg);
+  default:
+    throw std::runtime_error("gfx950 platform: " + msg);
+  }
+}
+
+enum class ProcessingContext { Evaluation, Simulation };
+
+namespace descriptors {
+
+/// D3Q19 (descriptor/definition/common.h:152-200)
+struct D3Q19 {
+  static constexpr int d = 3;
+  static constexpr int q = 19;
+  static constexpr int c[19][3] = {
+    {0, 0, 0},
+    {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {-1, -1, 0}, {-1, 1, 0}, {-1, 0, -1},
+    {-1, 0, 1}, {0, -1, -1}, {0, -1, 1},
+    {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, -1, 0}, {1, 0, 1},
+    {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
+  static constexpr int tden[19] = {3, 18, 18, 18, 36, 36, 36, 36, 36, 36,
+                                   18, 18, 18, 36, 36, 36, 36, 36, 36};
+};
+
+struct OMEGA { };
+struct FORCE { static constexpr int32_t field = OLBGFX_FIELD_FORCE; };
+struct VELOCITY { static constexpr int32_t field = OLBGFX_FIELD_VELOCITY; };
+
+}  // namespace descriptors
+
+namespace collision {
+struct TRT {
+  struct MAGIC { };
+};
+}  // namespace collision
+
+// ---- dynamics (dynamics/dynamics.h) --------------------------------------
+template <typename T, typename DESCRIPTOR> struct NoDynamics {
+  static constexpr int32_t kind = OLBGFX_NO_DYNAMICS;
+};
+template <typename T, typename DESCRIPTOR> struct BGKdynamics {
+  static constexpr int32_t kind = OLBGFX_BGK;
+};
+template <typename T, typename DESCRIPTOR> struct TRTdynamics {
+  static constexpr int32_t kind = OLBGFX_TRT;
+};
+template <typename T, typename DESCRIPTOR> struct ForcedBGKdynamics {
+  static constexpr int32_t kind = OLBGFX_FORCED_BGK;
+};
+template <typename T, typename DESCRIPTOR> struct ForcedTRTdynamics {
+  static constexpr int32_t kind = OLBGFX_FORCED_TRT;
+};
+template <typename T, typename DESCRIPTOR> struct BounceBack {
+  static constexpr int32_t kind = OLBGFX_BOUNCE_BACK;
+};
+template <typename T, typename DESCRIPTOR> struct BounceBackVelocity {
+  static constexpr int32_t kind = OLBGFX_BOUNCE_BACK_VELOCITY;
+};
+
+/// population storage of the platform
+enum class Storage : int32_t {
+  F32 = OLBGFX_STORE_F32,
+  F16 = OLBGFX_STORE_F16,
+  BF16 = OLBGFX_STORE_BF16,
+  F64 = OLBGFX_STORE_F64
+};
+
+// ---- geometry ------------------------------------------------------------
+
+/// Material numbers of one cuboid with its overlap, in CellID order
+/// (core/blockStructure.h:150-160); 0 marks cells outside the domain.
+template <typename T>
+class SuperGeometry {
+public:
+  SuperGeometry(std::array<int, 3> extent, int overlap = 3,
+                std::array<bool, 3> periodic = {false, false, false})
+    : _extent(extent), _overlap(overlap), _periodic(periodic)
+  {
+    if (extent[0] <= 0 || extent[1] <= 0 || extent[2] <= 0 || overlap < 1) {
+      throw std::invalid_argument("extent must be positive and the overlap at least 1");
+    }
+    for (int a = 0; a < 3; ++a) {
+      _size[a] = extent[a] + 2 * overlap;
+    }
+    _material.assign((std::size_t)_size[0] * _size[1] * _size[2], 0);
+  }
+
+  std::array<int, 3> getExtent() const { return _extent; }
+  int getOverlap() const { return _overlap; }
+  std::array<bool, 3> getPeriodicity() const { return _periodic; }
+  std::size_t getNcells() const { return _material.size(); }
+
+  CellID getCellId(int x, int y, int z) const
+  {
+    return (CellID)(((std::size_t)(x + _overlap) * _size[1] + (y + _overlap)) * _size[2] + (z + _overlap));
+  }
+  int get(int x, int y, int z) const { return _material[getCellId(x, y, z)]; }
+  void set(int x, int y, int z, int m) { _material[getCellId(x, y, z)] = (std::uint8_t)m; }
+
+  /// every core cell of material fromM becomes toM (blockGeometry.hh:401-408)
+  void rename(int fromM, int toM)
+  {
+    forCore([&](int x, int y, int z) {
+      if (get(x, y, z) == fromM) set(x, y, z, toM);
+    });
+  }
+  /// core cells of material fromM inside `indicator(x, y, z)` become toM
+  /// (blockGeometry.hh:410-434, lattice coordinates instead of physical ones)
+  void rename(int fromM, int toM, const std::function<bool(int, int, int)>& indicator)
+  {
+    forCore([&](int x, int y, int z) {
+      if (get(x, y, z) == fromM && indicator(x, y, z)) set(x, y, z, toM);
+    });
+  }
+  /// core cells of material fromM whose whole offset neighbourhood is fromM
+  /// become toM (blockGeometry.hh:435-466)
+  void rename(int fromM, int toM, std::array<int, 3> offset)
+  {
+    std::vector<CellID> hit;
+    forCore([&](int x, int y, int z) {
+      if (get(x, y, z) != fromM) return;
+      for (int i = -offset[0]; i <= offset[0]; ++i)
+        for (int j = -offset[1]; j <= offset[1]; ++j)
+          for (int k = -offset[2]; k <= offset[2]; ++k)
+            if (getClamped(x + i, y + j, z + k) != fromM) return;
+      hit.push_back(getCellId(x, y, z));
+    });
+    for (CellID c : hit) _material[c] = (std::uint8_t)toM;
+  }
+
+  /// cells with material != 0 (SuperGeometryStatistics::getNvoxel)
+  std::size_t getNvoxel() const
+  {
+    std::size_t n = 0;
+    forCore([&](int x, int y, int z) { n += get(x, y, z) != 0; });
+    return n;
+  }
+
+  const std::vector<std::uint8_t>& materials() const { return _material; }
+
+  template <typename F>
+  void forCore(F f) const
+  {
+    for (int x = 0; x < _extent[0]; ++x)
+      for (int y = 0; y < _extent[1]; ++y)
+        for (int z = 0; z < _extent[2]; ++z) f(x, y, z);
+  }
+
+private:
+  int getClamped(int x, int y, int z) const
+  {
+    if (x < -_overlap || y < -_overlap || z < -_overlap || x >= _extent[0] + _overlap ||
+        y >= _extent[1] + _overlap || z >= _extent[2] + _overlap) {
+      return 0;
+    }
+    return get(x, y, z);
+  }
+
+  std::array<int, 3> _extent;
+  int _overlap;
+  std::array<bool, 3> _periodic;
+  std::array<int, 3> _size{};
+  std::vector<std::uint8_t> _material;
+};
+
+// ---- statistics (core/latticeStatistics.h:43-127) -------------------------
+template <typename T>
+struct LatticeStatistics {
+  std::size_t numCells = 0;
+  T averageRho = 0;
+  T averageEnergy = 0;  // 0.5 u^2 (core/latticeStatistics.hh:89)
+  T maxU = 0;
+  std::size_t getNumCells() const { return numCells; }
+  T getAverageRho() const { return averageRho; }
+  T getAverageEnergy() const { return averageEnergy; }
+  T getMaxU() const { return maxU; }
+};
+
+// ---- the lattice ---------------------------------------------------------
+
+template <typename T, typename DESCRIPTOR = descriptors::D3Q19>
+class SuperLattice {
+  static_assert(std::is_same_v<DESCRIPTOR, descriptors::D3Q19>, "the gfx950 platform provides D3Q19");
+  static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>, "T is float or double");
+
+public:
+  explicit SuperLattice(SuperGeometry<T>& geometry,
+                        Storage storage = std::is_same_v<T, double> ? Storage::F64 : Storage::F32)
+    : _geometry(geometry), _storage(storage)
+  {
+    if ((storage == Storage::F64) != std::is_same_v<T, double>) {
+      throw std::invalid_argument("FP64 storage goes with T = double, the others with T = float");
+    }
+    const auto e = geometry.getExtent();
+    const auto p = geometry.getPeriodicity();
+    olbgfx_lattice_desc desc{};
+    desc.nx = e[0];
+    desc.ny = e[1];
+    desc.nz = e[2];
+    desc.padding = geometry.getOverlap();
+    for (int a = 0; a < 3; ++a) desc.periodic[a] = p[a] ? 1 : 0;
+    desc.storage = (int32_t)storage;
+    olbgfx_lattice* h = nullptr;
+    check(olbgfx_lattice_create(&desc, &h));
+    _handle.reset(h);
+    _ncells = geometry.getNcells();
+    _slotOfCell.assign(_ncells, 0);
+    _slots.push_back(Slot{OLBGFX_NO_DYNAMICS});
+  }
+
+  SuperLattice(const SuperLattice&) = delete;
+  SuperLattice& operator=(const SuperLattice&) = delete;
+
+  olbgfx_lattice* handle() { return _handle.get(); }
+  std::size_t getNcells() const { return _ncells; }
+
+  /// cells of `material` get DYNAMICS (core/superLattice.h:246-250)
+  template <template <typename, typename> class DYNAMICS>
+  void defineDynamics(SuperGeometry<T>& geometry, int material)
+  {
+    const int slot = slotFor(DYNAMICS<T, DESCRIPTOR>::kind);
+    forMaterial(geometry, material, [&](CellID c) { _slotOfCell[c] = (std::uint8_t)slot; });
+    _dirty = true;
+  }
+
+  /// set a dynamics parameter on every dynamics that has it
+  template <typename PARAMETER>
+  void setParameter(T value)
+  {
+    for (Slot& s : _slots) {
+      if constexpr (std::is_same_v<PARAMETER, descriptors::OMEGA>) {
+        if (s.kind != OLBGFX_NO_DYNAMICS && s.kind != OLBGFX_BOUNCE_BACK &&
+            s.kind != OLBGFX_BOUNCE_BACK_VELOCITY) {
+          s.omega = value;
+        }
+      } else if constexpr (std::is_same_v<PARAMETER, collision::TRT::MAGIC>) {
+        if (s.kind == OLBGFX_TRT || s.kind == OLBGFX_FORCED_TRT) {
+          s.magic = value;
+        }
+      } else {
+        static_assert(sizeof(PARAMETER) == 0, "parameter not provided by the gfx950 platform");
+      }
+    }
+    _dirty = true;
+  }
+
+  /// per-cell FORCE or VELOCITY on the cells of `material`
+  template <typename FIELD>
+  void defineField(SuperGeometry<T>& geometry, int material, std::array<T, 3> value)
+  {
+    auto& field = fieldOf(FIELD::field);
+    forMaterial(geometry, material, [&](CellID c) {
+      for (int a = 0; a < 3; ++a) field.values[3 * (std::size_t)c + a] = value[a];
+      field.set[c] = 1;
+    });
+    _dirty = true;
+  }
+
+  /// u of cells with externally stored velocity (momenta::setVelocity on
+  /// BounceBackVelocity cells writes their VELOCITY field)
+  void defineU(SuperGeometry<T>& geometry, int material, std::array<T, 3> u)
+  {
+    defineField<descriptors::VELOCITY>(geometry, material, u);
+  }
+
+  /// f_i = feq_i(rho(x), u(x)) on the cells of `material`
+  void iniEquilibrium(SuperGeometry<T>& geometry, int material,
+                      const std::function<T(int, int, int)>& rho,
+                      const std::function<std::array<T, 3>(int, int, int)>& u)
+  {
+    std::vector<std::uint8_t> sel(_ncells, 0);
+    std::vector<T> r(_ncells, T(1)), v(3 * _ncells, T(0));
+    geometry.forCore([&](int x, int y, int z) {
+      if (geometry.get(x, y, z) != material) return;
+      const CellID c = geometry.getCellId(x, y, z);
+      sel[c] = 1;
+      r[c] = rho(x, y, z);
+      const auto uu = u(x, y, z);
+      for (int a = 0; a < 3; ++a) v[3 * (std::size_t)c + a] = uu[a];
+    });
+    check(olbgfx_ini_equilibrium(handle(), sel.data(), valueType(), r.data(), v.data()));
+  }
+
+  /// upload dynamics, parameters and fields (core/superLattice.hh:248-266)
+  void initialize()
+  {
+    if (_slots.size() > OLBGFX_MAX_DYNAMICS) {
+      throw std::invalid_argument("too many distinct dynamics for one lattice");
+    }
+    std::vector<olbgfx_dynamics_desc> descs;
+    for (const Slot& s : _slots) descs.push_back(s.desc());
+    resolveField(OLBGFX_FIELD_FORCE, descs);
+    resolveField(OLBGFX_FIELD_VELOCITY, descs);
+    for (std::size_t k = 0; k < descs.size(); ++k) {
+      check(olbgfx_set_dynamics(handle(), (uint32_t)k, &descs[k]));
+    }
+    check(olbgfx_set_dynamics_mask(handle(), _slotOfCell.data()));
+    for (auto& kv : _fields) {
+      check(olbgfx_set_cell_field(handle(), kv.first, valueType(),
+                                  kv.second.uniform ? nullptr : kv.second.values.data()));
+    }
+    check(olbgfx_set_statistics(handle(), _statistics ? 1 : 0));
+    _dirty = false;
+    _initialized = true;
+  }
+
+  /// one step on the default stream (core/superLattice.hh:663-725)
+  void collideAndStream()
+  {
+    if (!_initialized || _dirty) {
+      initialize();
+    }
+    check(olbgfx_collide_and_stream(handle(), 1, nullptr));
+    _hostValid = false;
+  }
+
+  void setStatisticsOn() { setStatistics(true); }
+  void setStatisticsOff() { setStatistics(false); }
+
+  LatticeStatistics<T> getStatistics()
+  {
+    double s[4];
+    check(olbgfx_get_statistics(handle(), s));
+    LatticeStatistics<T> st;
+    st.numCells = (std::size_t)s[0];
+    if (s[0] > 0) {
+      st.averageRho = T(s[1] / s[0]);
+      st.averageEnergy = T(0.5 * s[2] / s[0]);
+      st.maxU = T(std::sqrt(s[3]));
+    }
+    return st;
+  }
+
+  /// Evaluation copies the populations into a host mirror in logical order
+  /// (core/platform/gpu/hip/column.hh:289-300)
+  void setProcessingContext(ProcessingContext context)
+  {
+    if (context == ProcessingContext::Evaluation) {
+      _host.resize((std::size_t)DESCRIPTOR::q * _ncells);
+      check(olbgfx_get_populations(handle(), valueType(), _host.data()));
+      _hostValid = true;
+    } else if (_hostValid) {
+      check(olbgfx_set_populations(handle(), valueType(), _host.data()));
+    }
+  }
+
+  /// population iPop of cell iCell, shifted by -t_i (evaluation mirror)
+  T& population(CellID iCell, int iPop)
+  {
+    if (!_hostValid) {
+      throw std::logic_error("call setProcessingContext(Evaluation) first");
+    }
+    return _host[(std::size_t)iPop * _ncells + iCell];
+  }
+
+  /// density and velocity of one cell from the evaluation mirror
+  void computeRhoU(CellID iCell, T& rho, T u[3])
+  {
+    T r = 0;
+    T j[3] = {0, 0, 0};
+    for (int i = 0; i < DESCRIPTOR::q; ++i) {
+      const T f = population(iCell, i);
+      r += f;
+      for (int a = 0; a < 3; ++a) j[a] += f * T(DESCRIPTOR::c[i][a]);
+    }
+    rho = r + T(1);
+    for (int a = 0; a < 3; ++a) u[a] = j[a] / rho;
+  }
+
+private:
+  struct Slot {
+    int32_t kind;
+    T omega = 0;
+    T magic = 0;
+    olbgfx_dynamics_desc desc() const
+    {
+      olbgfx_dynamics_desc d{};
+      d.kind = kind;
+      d.omega = (double)omega;
+      d.magic = (double)magic;
+      d.wall_rho = 1.0;
+      return d;
+    }
+  };
+  struct Field {
+    std::vector<T> values;
+    std::vector<std::uint8_t> set;
+    bool uniform = false;
+  };
+  struct Deleter {
+    void operator()(olbgfx_lattice* h) const { olbgfx_lattice_destroy(h); }
+  };
+
+  int32_t valueType() const { return std::is_same_v<T, double> ? OLBGFX_VALUE_F64 : OLBGFX_VALUE_F32; }
+
+  int slotFor(int32_t kind)
+  {
+    for (std::size_t k = 0; k < _slots.size(); ++k) {
+      if (_slots[k].kind == kind) return (int)k;
+    }
+    _slots.push_back(Slot{kind});
+    return (int)_slots.size() - 1;
+  }
+
+  Field& fieldOf(int32_t id)
+  {
+    Field& f = _fields[id];
+    if (f.values.empty()) {
+      f.values.assign(3 * _ncells, T(0));
+      f.set.assign(_ncells, 0);
+    }
+    return f;
+  }
+
+  template <typename F>
+  void forMaterial(SuperGeometry<T>& geometry, int material, F f)
+  {
+    for (CellID c = 0; c < (CellID)_ncells; ++c) {
+      if (geometry.materials()[c] == material) f(c);
+    }
+  }
+
+  // A field whose value is the same on every cell of each dynamics that has
+  // it (and that covers all of that dynamics' cells) goes into the dynamics
+  // table and costs no per-cell reads; otherwise it is uploaded per cell.
+  void resolveField(int32_t id, std::vector<olbgfx_dynamics_desc>& descs)
+  {
+    auto it = _fields.find(id);
+    if (it == _fields.end()) {
+      return;
+    }
+    Field& f = it->second;
+    const std::size_t ns = _slots.size();
+    std::vector<std::array<T, 3>> value(ns);
+    std::vector<char> seen(ns, 0), missing(ns, 0);
+    bool table = true;
+    for (CellID c = 0; c < (CellID)_ncells && table; ++c) {
+      const std::size_t k = _slotOfCell[c];
+      if (!f.set[c]) {
+        missing[k] = 1;
+        continue;
+      }
+      const std::array<T, 3> w{f.values[3 * (std::size_t)c], f.values[3 * (std::size_t)c + 1],
+                               f.values[3 * (std::size_t)c + 2]};
+      if (!seen[k]) {
+        value[k] = w;
+        seen[k] = 1;
+      } else if (w != value[k]) {
+        table = false;
+      }
+    }
+    for (std::size_t k = 0; k < ns && table; ++k) {
+      if (seen[k] && missing[k]) table = false;
+    }
+    f.uniform = table;
+    if (!table) {
+      return;
+    }
+    for (std::size_t k = 0; k < ns; ++k) {
+      if (!seen[k]) continue;
+      for (int a = 0; a < 3; ++a) {
+        if (id == OLBGFX_FIELD_FORCE) {
+          descs[k].force[a] = (double)value[k][a];
+        } else {
+          descs[k].wall_velocity[a] = (double)value[k][a];
+        }
+      }
+    }
+  }
+
+  void setStatistics(bool on)
+  {
+    _statistics = on;
+    check(olbgfx_set_statistics(handle(), on ? 1 : 0));
+  }
+
+  SuperGeometry<T>& _geometry;
+  Storage _storage;
+  std::unique_ptr<olbgfx_lattice, Deleter> _handle;
+  std::size_t _ncells = 0;
+  std::vector<Slot> _slots;
+  std::vector<std::uint8_t> _slotOfCell;
+  std::map<int32_t, Field> _fields;
+  std::vector<T> _host;
+  bool _hostValid = false;
+  bool _statistics = true;  // on by default (core/blockLattice.hh:42)
+  bool _dirty = true;
+  bool _initialized = false;
+};
+
+// ---- setters (case/setters.h:303-345, boundary/setBoundary.h:150-157) -----
+namespace dynamics {
+template <template <typename, typename> class DYNAMICS, typename T, typename DESCRIPTOR>
+void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material)
+{
+  lattice.template defineDynamics<DYNAMICS>(geometry, material);
+}
+}  // namespace dynamics
+
+namespace boundary {
+struct BounceBack {
+  template <typename T, typename DESCRIPTOR>
+  using dynamics = olb::BounceBack<T, DESCRIPTOR>;
+};
+template <typename BC, typename T, typename DESCRIPTOR>
+void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material)
+{
+  lattice.template defineDynamics<BC::template dynamics>(geometry, material);
+}
+}  // namespace boundary
+
+}  // namespace olb
+
+#endif  // OLB_GFX950_H

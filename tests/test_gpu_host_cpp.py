@@ -1,0 +1,45 @@
+"""The C++ host mirror (openlb_amd/host/olb_gfx950.h) drives the platform from
+a case written like OpenLB's cavity3dBenchmark; its result must match the
+oracle on the same case."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from openlb_amd import cases
+from oracle.oracle import OracleLattice
+from tests.helpers import rel_err_unshifted
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BINARY = os.path.join(ROOT, "openlb_amd", "host", "examples", "cavity3d")
+
+
+def test_cpp_cavity_matches_oracle(tmp_path):
+    if not os.path.exists(BINARY):
+        pytest.fail("openlb_amd/host/examples/cavity3d is not built (run __graft_entry__.build())")
+    res, steps = 16, 90
+    dump = tmp_path / "cavity.bin"
+    print("[cpp] cavity3d", res, steps, flush=True)
+    out = subprocess.run([BINARY, "--RESOLUTION", str(res), "--TIME_STEPS", str(steps), "--DUMP", str(dump)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    fields = out.stdout.strip().split(",")
+    assert int(fields[0]) == res and float(fields[-1]) > 0
+    c = cases.cavity3d(res, dtype=np.float32)
+    raw = np.fromfile(dump, dtype=np.uint8)
+    mats = raw[:c.ncells]
+    np.testing.assert_array_equal(mats, c.materials.reshape(-1))
+    f = raw[c.ncells:].view(np.float32).reshape(19, c.ncells)
+    orc = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(orc)
+    orc.collide_and_stream(10 + steps)
+    fluid = c.materials.reshape(-1) == 1
+    assert rel_err_unshifted(f[:, fluid], orc.get_populations()[:, fluid]) <= 1e-5
+
+
+def test_cpp_rejects_bad_options():
+    out = subprocess.run([BINARY, "--BOGUS", "1"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "unknown option" in out.stderr
