@@ -124,7 +124,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     tuned = None
     if tune:
         tuned = autotune.tune_ensemble(ens, steps=3, stream=handle, use_cache=False)
-        tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "ms")}
+        tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "group_bytes", "steps_per_group", "ticket", "ms")}
     ens.step(warmup, handle)
     steps = max(10, steps // 10 * 10)
     ms_persistent = ens.time("persistent", steps, handle)
@@ -132,6 +132,9 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     ens.launch_graph(1, handle)
     ms_graph = ens.time("graph", steps, handle)
     ms_sep = ens.time("separate", steps, handle)
+    ens.set_schedule(0, 1, False)
+    ens.step(2, handle)
+    ms_flat = ens.time("persistent", steps, handle)
     cells = ens.cells
     out = {
         "members": len(lats),
@@ -144,6 +147,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
         "graph_glups": cells * steps / (ms_graph * 1e-3) / 1e9,
         "persistent_glups": cells * steps / (ms_persistent * 1e-3) / 1e9,
         "separate_glups": cells * steps / (ms_sep * 1e-3) / 1e9,
+        "ungrouped_persistent_steps_per_s": steps / (ms_flat * 1e-3),
     }
     out["speedup_graph_vs_separate"] = ms_sep / ms_graph
     ens.close()

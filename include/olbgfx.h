@@ -176,6 +176,17 @@ olbgfx_status olbgfx_ensemble_destroy(olbgfx_ensemble* ensemble);
 olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* ensemble, uint32_t block_threads,
                                          uint32_t waves_per_simd, uint32_t tiles_per_block);
 
+/* schedule of ensemble steps: members are packed, in order, into groups of at
+ * most group_bytes of population storage (0 = all members in one group) and
+ * each group advances steps_per_group steps (1..1024) before the next group
+ * runs; members are independent, so every member still ends a call
+ * `steps` steps further.  A group that fits the 256 MB Infinity Cache is then
+ * re-read from it.  ticket = 1 lets the last block of each launch advance the
+ * ring offsets (one launch per group step); 0 uses a separate small kernel.
+ * Drops a captured graph.  Used by the autotuner. */
+olbgfx_status olbgfx_ensemble_set_schedule(olbgfx_ensemble* ensemble, uint64_t group_bytes,
+                                           uint32_t steps_per_group, int32_t ticket);
+
 /* advance every member by `steps` steps, one persistent launch per step */
 olbgfx_status olbgfx_ensemble_step(olbgfx_ensemble* ensemble, uint32_t steps, void* stream);
 

@@ -107,6 +107,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_ensemble_destroy": [vp],
         "olbgfx_ensemble_set_launch": [vp, u32, u32, u32],
         "olbgfx_ensemble_step": [vp, u32, vp],
+        "olbgfx_ensemble_set_schedule": [vp, ctypes.c_uint64, u32, i32],
         "olbgfx_ensemble_step_separately": [vp, u32, vp],
         "olbgfx_ensemble_capture": [vp, u32],
         "olbgfx_ensemble_launch_graph": [vp, u32, vp],
@@ -348,6 +349,13 @@ class Ensemble:
     def set_launch(self, block_threads: int = 256, waves_per_simd: int = 0, tiles_per_block: int = 1) -> None:
         _check(library().olbgfx_ensemble_set_launch(self._h, int(block_threads), int(waves_per_simd),
                                                      int(tiles_per_block)))
+
+    def set_schedule(self, group_bytes: int = 0, steps_per_group: int = 1, ticket: bool = False) -> None:
+        """pack members into groups of at most group_bytes of populations (0 =
+        one group); each group advances steps_per_group steps at a time; with
+        ticket the last block of a launch advances the ring offsets"""
+        _check(library().olbgfx_ensemble_set_schedule(self._h, int(group_bytes), int(steps_per_group),
+                                                       int(bool(ticket))))
 
     def step(self, steps: int = 1, stream: int | None = None) -> None:
         _check(library().olbgfx_ensemble_step(self._h, int(steps), stream))

@@ -79,24 +79,42 @@ def ensemble_key(extents, storage: str) -> str:
 
 
 def tune_ensemble(ens, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
-                  blocks=BLOCKS, waves=WAVES, tiles=TILES) -> dict:
-    """pick (block_threads, waves_per_simd, tiles_per_block) for an ensemble"""
+                  blocks=(128, 256), waves=(0, 3, 4, 5), tiles=(1, 2, 4),
+                  group_bytes=(0, 96 << 20, 160 << 20), steps_per_group=(1, 4, 8),
+                  tickets=(False, True)) -> dict:
+    """pick (block_threads, waves_per_simd, tiles_per_block), then the group
+    schedule (group_bytes, steps_per_group), for an ensemble"""
     key = key or ensemble_key([m.extent for m in ens.members], ens.members[0].storage)
     if use_cache:
         hit = _load().get(key)
         if hit:
             ens.set_launch(hit["block"], hit["waves"], hit["tiles"])
+            ens.set_schedule(hit["group_bytes"], hit["steps_per_group"], hit.get("ticket", False))
             return dict(hit, cached=True)
     trials = []
+    span = max(8, steps)
+    ens.set_schedule(0, 1, False)
     for b in blocks:
         for w in waves:
             for t in tiles:
                 ens.set_launch(b, w, t)
                 ens.step(1, stream)
-                ms = ens.time("persistent", steps, stream) / steps
-                trials.append({"block": b, "waves": w, "tiles": t, "ms": ms})
+                ms = ens.time("persistent", span, stream) / span
+                trials.append({"block": b, "waves": w, "tiles": t, "group_bytes": 0, "steps_per_group": 1,
+                               "ticket": False, "ms": ms})
     best = min(trials, key=lambda t: t["ms"])
     ens.set_launch(best["block"], best["waves"], best["tiles"])
-    _store(key, {k: best[k] for k in ("block", "waves", "tiles", "ms")})
-    return {"block": best["block"], "waves": best["waves"], "tiles": best["tiles"], "ms": best["ms"],
-            "trials": trials}
+    for gb in group_bytes:
+        for k in steps_per_group:
+            for tk in tickets:
+                ens.set_schedule(gb, k, tk)
+                ens.step(k, stream)
+                ms = ens.time("persistent", span, stream) / span
+                trials.append({"block": best["block"], "waves": best["waves"], "tiles": best["tiles"],
+                               "group_bytes": gb, "steps_per_group": k, "ticket": tk, "ms": ms})
+    best = min(trials, key=lambda t: t["ms"])
+    ens.set_launch(best["block"], best["waves"], best["tiles"])
+    ens.set_schedule(best["group_bytes"], best["steps_per_group"], best["ticket"])
+    keep = ("block", "waves", "tiles", "group_bytes", "steps_per_group", "ticket", "ms")
+    _store(key, {k: best[k] for k in keep})
+    return dict({k: best[k] for k in keep}, trials=trials)

@@ -47,6 +47,8 @@ constexpr int Q = 19;
 constexpr uint32_t kMaxBlock = 256;
 // LDS of one gfx950 compute unit
 constexpr uint32_t kLdsPerCu = 160 * 1024;
+// persistent blocks per ensemble launch at most (bounds the ticket atomics)
+constexpr uint32_t kMaxEnsembleBlocks = 8192;
 
 // D3Q19 velocity set, descriptor/definition/common.h:166-199.
 // Index 0 rests, 1..9 are the negative half, i+9 is the opposite of i.
@@ -542,22 +544,27 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
   const uint32_t z = r - y * (uint32_t)a.nz;
   const uint32_t n = ((x + a.pad) * a.py + (y + a.pad)) * a.pz + (z + a.pad);
 
+  // In the masked path the populations are loaded before the dynamics id is
+  // known, so their loads overlap the mask and table loads instead of
+  // waiting behind them; a NoDynamics core cell then stores nothing.
   int kind;
   int id;
+  A f[Q];
   if constexpr (MODE >= 0) {
     id = a.uniform_id;
     kind = MODE;
   } else {
     id = (int)a.mask[n];
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = St::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)));
+  }
+  if constexpr (MODE < 0) {
     kind = a.table[id].kind;
   }
   if (kind == OLBGFX_NO_DYNAMICS) {
     return;
-  }
-  A f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    f[i] = St::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)));
   }
 
   if constexpr (MODE >= 0) {
@@ -631,59 +638,92 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
 }
 
 // ---------------------------------------------------------------------------
-// the ensemble kernel: one persistent launch advances every member lattice by
-// one step.  The work is a list of tiles (member, first core cell) of
-// blockDim.x cells each; logical block lb owns the contiguous tile range
-// [lb*tiles_per_block, (lb+1)*tiles_per_block), with the XCD-aware order of
-// the single-lattice kernel.  Member descriptors and ring offsets live in
-// device memory, so the launch has fixed arguments and can be replayed from a
-// captured graph.
+// the ensemble kernel: one persistent launch advances one group of member
+// lattices by one step.  The work is a list of tiles (member, first core
+// cell) of blockDim.x cells each; logical block lb owns a contiguous range of
+// tiles_per_block tiles of the group, with the XCD-aware order of the
+// single-lattice kernel.  Member descriptors and ring offsets live in device
+// memory, so the launch has fixed arguments and can be replayed from a
+// captured graph.  The last block to finish (ticket counter) advances the
+// ring offsets of the group's members, s_i -= d_i (mod ncells), for the next
+// launch; the counter is left at zero for it.
 
 struct Tile {
   uint32_t member;
   uint32_t first;
 };
 
+struct GroupArgs {
+  uint32_t ticket;           // 1: the last block advances the offsets (else a separate kernel)
+  uint32_t tile_begin;       // first tile of the group
+  uint32_t ntiles;           // tiles of the group
+  uint32_t member_begin;     // members [member_begin, member_end) form the group
+  uint32_t member_end;
+  uint32_t tiles_per_block;
+  uint32_t nblocks;          // logical blocks that carry tiles
+  uint32_t blocks_per_xcd;
+  uint32_t* counter;         // ticket counter of the group (0 between launches)
+};
+
 template <typename S>
 __global__ void __launch_bounds__(kMaxBlock)
     ensemble_kernel(const StepArgs<typename Storage<S>::A>* members, const Tile* tiles,
-                    uint32_t ntiles, uint32_t tiles_per_block, uint32_t nblocks,
-                    uint32_t blocks_per_xcd)
+                    const GroupArgs g, uint32_t* const* shifts, const uint32_t* dmod,
+                    const uint32_t* counts)
 {
   using A = typename Storage<S>::A;
   const uint32_t b = blockIdx.x;
-  const uint32_t lb = (b & 7u) * blocks_per_xcd + (b >> 3);
-  if (lb >= nblocks) {
+  const uint32_t lb = (b & 7u) * g.blocks_per_xcd + (b >> 3);
+  if (lb < g.nblocks) {
+    const uint32_t end = g.tile_begin + g.ntiles;
+    const uint32_t t0 = g.tile_begin + lb * g.tiles_per_block;
+    const uint32_t t1 = min(t0 + g.tiles_per_block, end);
+    for (uint32_t t = t0; t < t1; ++t) {
+      const Tile tile = tiles[t];
+      const StepArgs<A>& a = members[tile.member];
+      uint32_t sh[Q];
+#pragma unroll
+      for (int i = 0; i < Q; ++i) sh[i] = a.shift_dev[i];
+      const uint32_t lin = tile.first + threadIdx.x;
+      A st_rho, st_usqr;
+      bool counted = false;
+      if (lin < a.ncore) {
+        step_cell<S, -1>(a, sh, lin, st_rho, st_usqr, counted);
+      }
+    }
+  }
+  if (!g.ticket) {
     return;
   }
-  const uint32_t t0 = lb * tiles_per_block;
-  const uint32_t t1 = min(t0 + tiles_per_block, ntiles);
-  for (uint32_t t = t0; t < t1; ++t) {
-    const Tile tile = tiles[t];
-    const StepArgs<A>& a = members[tile.member];
-    uint32_t sh[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) sh[i] = a.shift_dev[i];
-    const uint32_t lin = tile.first + threadIdx.x;
-    A st_rho, st_usqr;
-    bool counted = false;
-    if (lin < a.ncore) {
-      step_cell<S, -1>(a, sh, lin, st_rho, st_usqr, counted);
+  // every block has read the offsets it needs before it takes its ticket, so
+  // the last one may advance them; the kernel boundary publishes the update
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    last = atomicAdd(g.counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) {
+    for (uint32_t k = g.member_begin * Q + threadIdx.x; k < g.member_end * Q; k += blockDim.x) {
+      const uint32_t l = k / Q, i = k % Q;
+      const uint64_t s = (uint64_t)shifts[l][i] + dmod[k];
+      shifts[l][i] = (uint32_t)(s >= counts[l] ? s - counts[l] : s);
+    }
+    if (threadIdx.x == 0) {
+      atomicExch(g.counter, 0u);
     }
   }
 }
 
-// advance the device ring offsets of `count` lattices by one step:
-// s_i -= d_i (mod ncells); dmod holds (-d_i) mod ncells per lattice
+// advance the ring offsets of members [begin, end): s_i -= d_i (mod ncells)
 __global__ void advance_shifts(uint32_t* const* shifts, const uint32_t* dmod, const uint32_t* counts,
-                               uint32_t nlat)
+                               uint32_t begin, uint32_t end)
 {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nlat * Q) {
+  const uint32_t k = begin * Q + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < end * Q) {
     const uint32_t l = k / Q, i = k % Q;
-    uint32_t* sh = shifts[l];
-    const uint64_t s = (uint64_t)sh[i] + dmod[k];
-    sh[i] = (uint32_t)(s >= counts[l] ? s - counts[l] : s);
+    const uint64_t s = (uint64_t)shifts[l][i] + dmod[k];
+    shifts[l][i] = (uint32_t)(s >= counts[l] ? s - counts[l] : s);
   }
 }
 
@@ -1617,6 +1657,18 @@ struct olbgfx_ensemble {
   size_t lds = 0;
   std::vector<uint64_t> versions;
   uint32_t tiled_block = 0;
+  // schedule: members are packed into groups of at most group_bytes of
+  // population storage (0 = one group), and each group advances
+  // steps_per_group steps before the next group runs, so a group that fits
+  // the 256 MB Infinity Cache is re-read from it instead of from HBM
+  uint64_t group_bytes = 0;
+  uint32_t steps_per_group = 1;
+  uint32_t ticket = 0;               // 1: last block advances the offsets; 0: separate kernel
+  uint64_t grouped_bytes = ~0ull;
+  std::vector<uint32_t> order;       // member index of each descriptor slot
+  std::vector<GroupArgs> groups;
+  uint32_t* counters_dev = nullptr;
+  uint32_t counters_capacity = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture_stream = nullptr;
@@ -1656,7 +1708,7 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
 {
   using A = typename Storage<S>::A;
   const size_t m = E->members.size();
-  bool changed = force || E->tiled_block != E->block_threads;
+  bool changed = force || E->tiled_block != E->block_threads || E->grouped_bytes != E->group_bytes;
   for (size_t k = 0; k < m; ++k) {
     changed = changed || E->versions[k] != E->members[k]->version;
   }
@@ -1670,22 +1722,49 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
   if (!changed) {
     return OLBGFX_OK;
   }
-  std::vector<StepArgs<A>> args(m);
-  std::vector<Tile> tiles;
-  std::vector<uint32_t*> shift_ptrs(m);
-  std::vector<uint32_t> dmod(m * Q), counts(m);
-  for (size_t k = 0; k < m; ++k) {
-    olbgfx_lattice* L = E->members[k];
-    fill_args(L, args[k], true);
-    args[k].uniform_id = -1;
-    args[k].stats = nullptr;
-    shift_ptrs[k] = L->shift_dev;
-    counts[k] = L->count;
-    for (int i = 0; i < Q; ++i) dmod[k * Q + i] = (uint32_t)L->dmod[i];
-    for (uint32_t first = 0; first < args[k].ncore; first += E->block_threads) {
-      tiles.push_back(Tile{(uint32_t)k, first});
+  // groups: members in the given order, a new group whenever the next member
+  // would push the group's population storage over group_bytes
+  std::vector<std::vector<uint32_t>> packs;
+  uint64_t bytes = 0;
+  for (uint32_t k = 0; k < (uint32_t)m; ++k) {
+    const olbgfx_lattice* L = E->members[k];
+    const uint64_t b = (uint64_t)Q * L->stride * L->elem;
+    if (packs.empty() || (E->group_bytes > 0 && bytes + b > E->group_bytes && !packs.back().empty())) {
+      packs.emplace_back();
+      bytes = 0;
     }
-    E->versions[k] = L->version;
+    packs.back().push_back(k);
+    bytes += b;
+  }
+  std::vector<StepArgs<A>> args;
+  std::vector<Tile> tiles;
+  std::vector<uint32_t*> shift_ptrs;
+  std::vector<uint32_t> dmod, counts;
+  E->order.clear();
+  E->groups.clear();
+  for (const auto& pack : packs) {
+    GroupArgs g{};
+    g.tile_begin = (uint32_t)tiles.size();
+    g.member_begin = (uint32_t)args.size();
+    for (uint32_t k : pack) {
+      olbgfx_lattice* L = E->members[k];
+      const uint32_t slot = (uint32_t)args.size();
+      args.emplace_back();
+      fill_args(L, args.back(), true);
+      args.back().uniform_id = -1;
+      args.back().stats = nullptr;
+      shift_ptrs.push_back(L->shift_dev);
+      counts.push_back(L->count);
+      for (int i = 0; i < Q; ++i) dmod.push_back((uint32_t)L->dmod[i]);
+      for (uint32_t first = 0; first < args.back().ncore; first += E->block_threads) {
+        tiles.push_back(Tile{slot, first});
+      }
+      E->order.push_back(k);
+      E->versions[k] = L->version;
+    }
+    g.member_end = (uint32_t)args.size();
+    g.ntiles = (uint32_t)tiles.size() - g.tile_begin;
+    E->groups.push_back(g);
   }
   if (!E->args_dev) {
     HIP_TRY(hipMalloc(&E->args_dev, sizeof(StepArgs<A>) * m));
@@ -1700,6 +1779,20 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
     HIP_TRY(hipMalloc(&E->tiles_dev, sizeof(Tile) * tiles.size()));
     E->tiles_capacity = (uint32_t)tiles.size();
   }
+  if (E->groups.size() > E->counters_capacity) {
+    if (E->counters_dev) {
+      HIP_TRY(hipFree(E->counters_dev));
+    }
+    HIP_TRY(hipMalloc(&E->counters_dev, sizeof(uint32_t) * E->groups.size()));
+    E->counters_capacity = (uint32_t)E->groups.size();
+  }
+  // counters start at zero; the kernels leave them at zero (no launch of
+  // this ensemble may be in flight while they are reset)
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemset(E->counters_dev, 0, sizeof(uint32_t) * E->groups.size()));
+  for (size_t gi = 0; gi < E->groups.size(); ++gi) {
+    E->groups[gi].counter = E->counters_dev + gi;
+  }
   HIP_TRY(hipMemcpy(E->args_dev, args.data(), sizeof(StepArgs<A>) * m, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(E->tiles_dev, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(E->shift_ptrs_dev, shift_ptrs.data(), sizeof(uint32_t*) * m, hipMemcpyHostToDevice));
@@ -1707,24 +1800,41 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
   HIP_TRY(hipMemcpy(E->counts_dev, counts.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
   E->ntiles = (uint32_t)tiles.size();
   E->tiled_block = E->block_threads;
+  E->grouped_bytes = E->group_bytes;
   return OLBGFX_OK;
 }
 
-// enqueue `steps` ensemble steps (persistent kernel + offset advance)
+// enqueue `steps` ensemble steps: in rounds of steps_per_group steps, each
+// group advances that many steps (one persistent launch per step) before the
+// next group runs
 template <typename S>
 olbgfx_status ensemble_enqueue(olbgfx_ensemble* E, uint32_t steps, hipStream_t stream)
 {
   using A = typename Storage<S>::A;
-  const uint32_t nblocks = (E->ntiles + E->tiles_per_block - 1) / E->tiles_per_block;
-  const uint32_t per_xcd = (nblocks + 7) / 8;
-  const dim3 grid(8 * per_xcd), block(E->block_threads);
-  const uint32_t m = (uint32_t)E->members.size();
-  const dim3 agrid((m * Q + 255) / 256), ablock(256);
-  for (uint32_t k = 0; k < steps; ++k) {
-    ensemble_kernel<S><<<grid, block, E->lds, stream>>>((const StepArgs<A>*)E->args_dev, E->tiles_dev,
-                                                        E->ntiles, E->tiles_per_block, nblocks,
-                                                        per_xcd);
-    advance_shifts<<<agrid, ablock, 0, stream>>>(E->shift_ptrs_dev, E->dmod_dev, E->counts_dev, m);
+  uint32_t done = 0;
+  while (done < steps) {
+    const uint32_t k = std::min(E->steps_per_group, steps - done);
+    for (GroupArgs g : E->groups) {
+      // at most kMaxEnsembleBlocks persistent blocks per launch
+      g.tiles_per_block = std::max(E->tiles_per_block,
+                                   (g.ntiles + kMaxEnsembleBlocks - 1) / kMaxEnsembleBlocks);
+      g.nblocks = (g.ntiles + g.tiles_per_block - 1) / g.tiles_per_block;
+      g.blocks_per_xcd = (g.nblocks + 7) / 8;
+      g.ticket = E->ticket;
+      const dim3 grid(8 * g.blocks_per_xcd), block(E->block_threads);
+      const uint32_t nm = (g.member_end - g.member_begin) * Q;
+      for (uint32_t j = 0; j < k; ++j) {
+        ensemble_kernel<S><<<grid, block, E->lds, stream>>>((const StepArgs<A>*)E->args_dev,
+                                                            E->tiles_dev, g, E->shift_ptrs_dev,
+                                                            E->dmod_dev, E->counts_dev);
+        if (!E->ticket) {
+          advance_shifts<<<(nm + 255) / 256, 256, 0, stream>>>(E->shift_ptrs_dev, E->dmod_dev,
+                                                               E->counts_dev, g.member_begin,
+                                                               g.member_end);
+        }
+      }
+    }
+    done += k;
   }
   HIP_TRY(hipGetLastError());
   return OLBGFX_OK;
@@ -1806,6 +1916,7 @@ olbgfx_status olbgfx_ensemble_destroy(olbgfx_ensemble* E)
   (void)hipFree(E->shift_ptrs_dev);
   (void)hipFree(E->dmod_dev);
   (void)hipFree(E->counts_dev);
+  (void)hipFree(E->counters_dev);
   delete E;
   return OLBGFX_OK;
 }
@@ -1829,6 +1940,22 @@ olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* E, uint32_t block_thre
   E->waves_per_simd = waves_per_simd;
   E->tiles_per_block = tiles_per_block;
   E->lds = lds_for(block_threads, waves_per_simd);
+  drop_graph(E);
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_set_schedule(olbgfx_ensemble* E, uint64_t group_bytes,
+                                           uint32_t steps_per_group, int32_t ticket)
+{
+  if (!E) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble");
+  }
+  if (steps_per_group == 0 || steps_per_group > 1024) {
+    return fail(OLBGFX_ERR_INVALID, "steps per group must be in [1, 1024]");
+  }
+  E->group_bytes = group_bytes;
+  E->steps_per_group = steps_per_group;
+  E->ticket = ticket ? 1 : 0;
   drop_graph(E);
   return OLBGFX_OK;
 }

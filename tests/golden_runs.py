@@ -130,3 +130,22 @@ def velocity_error_l2(lat, meta, d) -> float:
     u_phys = u.T * meta["u_factor"]
     sol = d["usol_fluid"][sel]
     return float(np.sqrt(np.sum((sol - u_phys) ** 2) / np.sum(sol ** 2)))
+
+
+def cavity_reduced(lat, meta, d):
+    """mid-plane velocity, sample populations and fluid sums of a cavity run,
+    in the layout of the cavity3d_res64 fixture"""
+    from tests.helpers import C
+
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    f = lat.get_populations().astype(np.float64)
+    rho = f.sum(0) + 1.0
+    u = (C.T @ f / rho).T.reshape(shape + (3,))
+    mats = d["materials"].reshape(shape)
+    fluid = (mats == 1).reshape(-1)
+    zc = p + meta["res"] // 2
+    mid = u[p:-p, p:-p, zc, :]
+    uf = u.reshape(-1, 3)[fluid]
+    sums = np.array([(rho[fluid] - 1).sum(), *uf.sum(0), (uf ** 2).sum()])
+    return mid, f[:, d["sample_cells"]], sums

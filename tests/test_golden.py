@@ -71,3 +71,24 @@ def test_oracle_poiseuille_known_answer():
     sel = core_fluid(meta, d)
     f1 = lat.get_populations()[:, d["fluid"][sel]]
     assert rel_err_unshifted(f1, d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_oracle_cavity_res64_1000_steps_matches_reference():
+    """config 1 at full size: cavity3dBenchmark geometry, RESOLUTION 64,
+    BounceBack + BounceBackVelocity lid u = 0.1, Re = 100, 1000 steps, FP32"""
+    from tests.golden_runs import cavity_reduced
+
+    meta, d = load("cavity3d_res64_1000steps_f32")
+    c = cases.cavity3d(64, dtype=np.float32)
+    np.testing.assert_array_equal(c.materials.reshape(-1), d["materials"])
+    lat = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(lat)
+    lat.collide_and_stream(meta["steps"])
+    mid, fs, sums = cavity_reduced(lat, meta, d)
+    fluid = c.materials[3:-3, 3:-3, 3 + 32] == 1  # wall cells hold values streamed in from the padding
+    assert np.max(np.abs(mid - d["u_midplane"])[fluid]) <= 1e-6
+    assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5
+    # FP32 sums over 250k fluid cells: sum u^2 to 1e-4, the cancelling sums
+    # of u_x, u_y to 1e-2 (4e-8 per cell)
+    np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=1e-4)
+    np.testing.assert_allclose(sums[1:3], d["fluid_sums"][1:3], atol=1e-2)

@@ -63,6 +63,25 @@ def test_graph_with_tuned_launch_shape():
         np.testing.assert_array_equal(x.get_populations(), y.get_populations())
 
 
+@pytest.mark.parametrize("ticket", [False, True])
+@pytest.mark.parametrize("group_bytes,steps_per_group", [(1, 1), (1, 3), (200_000, 4), (0, 5)])
+def test_grouped_schedule_equals_members_stepped_alone(group_bytes, steps_per_group, ticket):
+    """members packed into groups, each advancing several steps at a time:
+    every member still ends exactly `steps` steps further"""
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ens = Ensemble(a)
+    ens.set_schedule(group_bytes, steps_per_group, ticket)
+    ens.step(7)
+    ens.capture(6)
+    ens.launch_graph(2)
+    for lat in b:
+        lat.collide_and_stream(19)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+        np.testing.assert_array_equal(x.ring_shifts(), y.ring_shifts())
+
+
 def test_interleaved_ensemble_and_single_steps():
     _, a = _members("f32")
     _, b = _members("f32")
@@ -129,7 +148,8 @@ def test_autotuners_keep_results(tmp_path, monkeypatch):
     b[0].collide_and_stream(4 * (1 + 2))  # four trials of one untimed + two timed steps
     np.testing.assert_array_equal(a[0].get_populations(), b[0].get_populations())
     ens = Ensemble(a)
-    r2 = autotune.tune_ensemble(ens, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2))
-    assert len(r2["trials"]) == 4
+    r2 = autotune.tune_ensemble(ens, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2),
+                                group_bytes=(0, 1), steps_per_group=(1, 2), tickets=(False,))
+    assert len(r2["trials"]) == 4 + 4
     cached = autotune.tune_lattice(a[0], steps=2)
     assert cached.get("cached")

@@ -59,3 +59,28 @@ def test_gpu_poiseuille_known_answer():
     assert abs(err - 0.158042) <= 1e-6
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+@pytest.mark.parametrize("storage,tol_u", [("f32", 1e-6), ("f16", 2e-4)])
+def test_gpu_cavity_res64_1000_steps_matches_reference(storage, tol_u):
+    """config 1 at full size on the GPU: cavity3dBenchmark geometry,
+    RESOLUTION 64, 1000 steps; FP32 against the reference's FP32 run (mid-plane
+    velocity within 1e-6 absolute, sample populations within 1e-5), FP16
+    shifted storage within 2e-4 absolute (u_lid = 0.1)"""
+    from openlb_amd import cases
+    from tests.golden_runs import cavity_reduced
+
+    meta, d = load("cavity3d_res64_1000steps_f32")
+    c = cases.cavity3d(64, dtype=np.float32)
+    lat = Lattice(c.extent, c.padding, c.periodic, storage)
+    c.apply(lat)
+    lat.collide_and_stream(meta["steps"])
+    mid, fs, sums = cavity_reduced(lat, meta, d)
+    fluid = c.materials[3:-3, 3:-3, 3 + 32] == 1  # wall cells hold values streamed in from the padding
+    du = np.max(np.abs(mid - d["u_midplane"])[fluid])
+    print("cavity64", storage, "max |du| mid-plane", du)
+    assert du <= tol_u
+    if storage == "f32":
+        assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5
+        np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=1e-4)
+        np.testing.assert_allclose(sums[1:3], d["fluid_sums"][1:3], atol=1e-2)
