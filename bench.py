@@ -67,7 +67,7 @@ def time_storage(torch, size, storage, steps, warmup, block, tune=True):
     tuned = None
     if tune and not block:
         tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
-        tuned = {k: tuned[k] for k in ("block", "waves", "ms")}
+        tuned = {k: tuned[k] for k in ("block", "waves", "order", "ms")}
     lat.collide_and_stream(warmup, handle)
     torch.cuda.synchronize()
 

@@ -159,6 +159,12 @@ olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* lattice, int32_t value_type, 
 olbgfx_status olbgfx_set_launch(olbgfx_lattice* lattice, uint32_t block_threads,
                                 uint32_t waves_per_simd);
 
+/* order in which the step kernel's blocks visit the cells: -1 (default) gives
+ * each of the 8 XCDs one contiguous part of the block; k in [0, 12] lets the
+ * XCDs take turns on runs of 2^k consecutive blocks.  Results do not depend
+ * on it.  Used by the autotuner; no reference counterpart. */
+olbgfx_status olbgfx_set_block_order(olbgfx_lattice* lattice, int32_t xcd_run_log2);
+
 /* ---- ensembles of independent lattices ------------------------------------
  * A batch of lattices (same storage type, any extents and dynamics) advanced
  * together: one persistent kernel per step walks a tile list that spans every

@@ -102,6 +102,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_reduce_moments": [vp, ctypes.POINTER(ctypes.c_double), vp],
         "olbgfx_compute_rho_u": [vp, i32, vp, vp],
         "olbgfx_set_launch": [vp, u32, u32],
+        "olbgfx_set_block_order": [vp, i32],
         "olbgfx_synchronize": [vp],
         "olbgfx_ensemble_create": [ctypes.POINTER(vp), u32, ctypes.POINTER(vp)],
         "olbgfx_ensemble_destroy": [vp],
@@ -284,7 +285,13 @@ class Lattice:
         (0 = no cap)"""
         _check(library().olbgfx_set_launch(self._h, int(block_threads), int(waves_per_simd)))
 
+    def set_block_order(self, xcd_run_log2: int = -1) -> None:
+        """-1: each XCD sweeps one contiguous part; k: XCDs take turns on runs
+        of 2^k blocks (results do not change)"""
+        _check(library().olbgfx_set_block_order(self._h, int(xcd_run_log2)))
+
     # -- statistics -----------------------------------------------------------------
+
     def set_statistics(self, on: bool) -> None:
         _check(library().olbgfx_set_statistics(self._h, int(bool(on))))
 

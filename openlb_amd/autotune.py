@@ -16,6 +16,7 @@ import json
 import os
 
 BLOCKS = (64, 128, 256)
+ORDERS = (-1, 0, 2, 4, 6)  # block orders: contiguous per XCD, runs of 2^k blocks
 WAVES = (0, 2, 3, 4, 5, 6)
 TILES = (1, 2, 4, 8)
 
@@ -51,26 +52,39 @@ def lattice_key(lat, mask_digest: str = "") -> str:
 
 
 def tune_lattice(lat, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
-                 blocks=BLOCKS, waves=WAVES) -> dict:
-    """pick (block_threads, waves_per_simd) for one lattice and apply it"""
+                 blocks=BLOCKS, waves=WAVES, orders=ORDERS) -> dict:
+    """pick (block_threads, waves_per_simd), then the block order, for one
+    lattice and apply them"""
     key = key or lattice_key(lat)
     if use_cache:
         hit = _load().get(key)
         if hit:
             lat.set_launch(hit["block"], hit["waves"])
+            lat.set_block_order(hit.get("order", -1))
             return dict(hit, cached=True)
     trials = []
+    lat.set_block_order(-1)
     for b in blocks:
         for w in waves:
             lat.set_launch(b, w)
             lat.collide_and_stream(1, stream)
             ms = lat.time_steps(steps, stream) / steps
-            trials.append({"block": b, "waves": w, "ms": ms})
+            trials.append({"block": b, "waves": w, "order": -1, "ms": ms})
     best = min(trials, key=lambda t: t["ms"])
     lat.set_launch(best["block"], best["waves"])
-    result = {"block": best["block"], "waves": best["waves"], "ms": best["ms"], "trials": trials}
-    _store(key, {"block": best["block"], "waves": best["waves"], "ms": best["ms"]})
-    return result
+    for o in orders:
+        if o == -1:
+            continue
+        lat.set_block_order(o)
+        lat.collide_and_stream(1, stream)
+        ms = lat.time_steps(steps, stream) / steps
+        trials.append({"block": best["block"], "waves": best["waves"], "order": o, "ms": ms})
+    best = min(trials, key=lambda t: t["ms"])
+    lat.set_launch(best["block"], best["waves"])
+    lat.set_block_order(best["order"])
+    pick = {k: best[k] for k in ("block", "waves", "order", "ms")}
+    _store(key, pick)
+    return dict(pick, trials=trials)
 
 
 def ensemble_key(extents, storage: str) -> str:

@@ -161,7 +161,8 @@ struct StepArgs {
   uint64_t magic_nz;         // ceil(2^64 / nz),  0 if nz == 1
   uint32_t ncore;            // nx*ny*nz
   uint32_t nblocks;          // blocks that carry cells
-  uint32_t blocks_per_xcd;   // ceil(nblocks / 8)
+  uint32_t blocks_per_xcd;   // ceil(nblocks / 8), rounded up to whole runs
+  int32_t xcd_run_log2;      // <0: one contiguous part per XCD; k: runs of 2^k blocks
   int32_t periodic;          // bit a set: axis a periodic
   const A* cell_u;           // optional VELOCITY field (3 per cell)
   const A* cell_rho;         // optional RHO field
@@ -606,10 +607,19 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
   using A = typename Storage<S>::A;
 
   // XCD-aware order: the hardware deals consecutive workgroups round-robin to
-  // the 8 XCDs; map them so each XCD sweeps one contiguous part of the block
-  // and the cache lines shared by neighbouring segments stay in one L2.
+  // the 8 XCDs.  By default each XCD sweeps one contiguous part of the block,
+  // so the cache lines shared by neighbouring segments stay in one L2; with
+  // xcd_run_log2 = k the XCDs take turns on runs of 2^k consecutive blocks,
+  // which keeps the cells in flight on the whole chip close together.  Both
+  // maps are bijections of [0, 8 * blocks_per_xcd).
   const uint32_t b = blockIdx.x;
-  const uint32_t lb = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  uint32_t lb;
+  if (a.xcd_run_log2 < 0) {
+    lb = (b & 7u) * a.blocks_per_xcd + (b >> 3);
+  } else {
+    const uint32_t k = (uint32_t)a.xcd_run_log2, g = b >> 3;
+    lb = ((g >> k) << (k + 3)) + ((b & 7u) << k) + (g & ((1u << k) - 1u));
+  }
   const uint32_t lin = lb * blockDim.x + threadIdx.x;
   const bool in_grid = lb < a.nblocks;
   const bool active = in_grid && lin < a.ncore;
@@ -933,6 +943,7 @@ struct olbgfx_lattice {
   int32_t uniform_id = 0;     // all cells start with id 0 = NoDynamics
   uint32_t block_threads = 256;
   uint32_t lds_reserve = 0;    // dynamic LDS per block that limits occupancy
+  int32_t xcd_run_log2 = -1;   // block order of the step kernel (olbgfx_set_block_order)
   olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
   std::vector<uint8_t> host_mask;
   uint64_t version = 0;        // bumped by every change of dynamics, mask or fields
@@ -987,6 +998,11 @@ olbgfx_status fill_args(olbgfx_lattice* L, StepArgs<A>& a, bool device_shifts)
   a.ncore = (uint32_t)D.nx * a.nyz;
   a.nblocks = (a.ncore + L->block_threads - 1) / L->block_threads;
   a.blocks_per_xcd = (a.nblocks + 7) / 8;
+  a.xcd_run_log2 = L->xcd_run_log2;
+  if (a.xcd_run_log2 >= 0) {
+    const uint32_t run = 1u << a.xcd_run_log2;
+    a.blocks_per_xcd = (a.blocks_per_xcd + run - 1) / run * run;
+  }
   a.periodic = (D.periodic[0] ? 1 : 0) | (D.periodic[1] ? 2 : 0) | (D.periodic[2] ? 4 : 0);
   a.cell_u = (const A*)L->cell_u;
   a.cell_rho = (const A*)L->cell_rho;
@@ -1624,6 +1640,18 @@ olbgfx_status olbgfx_set_launch(olbgfx_lattice* L, uint32_t block_threads, uint3
   }
   L->block_threads = block_threads;
   L->lds_reserve = lds;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_block_order(olbgfx_lattice* L, int32_t xcd_run_log2)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (xcd_run_log2 < -1 || xcd_run_log2 > 12) {
+    return fail(OLBGFX_ERR_INVALID, "block order must be -1 or a run length 2^k with k in [0, 12]");
+  }
+  L->xcd_run_log2 = xcd_run_log2;
   return OLBGFX_OK;
 }
 

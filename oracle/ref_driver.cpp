@@ -253,7 +253,15 @@ int run_porous(int nx, int ny, int nz, const std::string& spheres, double radius
     for (int ix = -1; ix <= 1; ++ix) {
       for (int iy = -1; iy <= 1; ++iy) {
         for (int iz = -1; iz <= 1; ++iz) {
-          Vector<T,3> c(T(cx + ix * ext[0]), T(cy + iy * ext[1]), T(cz + iz * ext[2]));
+          const double img[3] = {cx + ix * ext[0], cy + iy * ext[1], cz + iz * ext[2]};
+          bool touches = true;  // images clear of the block and its overlap rename nothing
+          for (int a = 0; a < 3; ++a) {
+            touches = touches && img[a] + radius >= -4.0 && img[a] - radius <= ext[a] + 3.0;
+          }
+          if (!touches) {
+            continue;
+          }
+          const Vector<T,3> c{T(img[0]), T(img[1]), T(img[2])};
           IndicatorSphere3D<T> s(c, T(radius));
           G->rename(1, 2, s);
         }

@@ -2,7 +2,8 @@
 path (oracle/_ref, built by `make -C oracle ref` from the OpenLB sources).
 
 Run in the build container (the OpenLB sources are not on the GPU host):
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py            # the small fixtures
+    python tests/golden/make_fixtures.py NAME ...   # full-size ones (FULL)
 Each fixture holds the materials of the padded block (CellID order), the
 neighbour distances, the populations at t = 0 and after the run for the core
 cells (logical order, shifted by -t_i), and the run's parameters.
@@ -30,9 +31,9 @@ POROUS = {"extent": (16, 12, 12), "radius": 2.5, "porosity": 0.7, "seed": 7, "ta
           "magic": 3.0 / 16.0, "fx": 1e-4, "steps": 200}
 
 
-def run(binary: str, args: list[str]) -> None:
+def run(binary: str, args: list[str], threads: int = 4) -> None:
     os.makedirs(RUN, exist_ok=True)
-    env = dict(os.environ, OMP_NUM_THREADS="4")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     subprocess.run([os.path.join(REF, binary), *args], cwd=RUN, env=env, check=True,
                    stdout=subprocess.DEVNULL)
 
@@ -99,7 +100,66 @@ def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000) -> N
     print("wrote", f"cavity3d_res{res}_{steps}steps_{tag}", sums)
 
 
+# full-size workloads, kept as reductions (tests/golden_runs.reduce_state)
+FULL = {
+    # periodic Taylor-Green 256^3, u0 = 0.04, tau = 0.56, 500 steps
+    "tgv256_500steps_f32": {"case": "tgv", "n": 256, "steps": 500, "stride": 4},
+    # porous channel 384x128x128, spheres r = 6 to porosity 0.7 (seed 20261202),
+    # ForcedTRT Lambda = 3/16, tau = 0.8, g = 1e-6, 5000 steps from rest
+    "porous384x128x128_5000steps_f32": {"case": "porous", "extent": (384, 128, 128), "radius": 6.0,
+                                        "porosity": 0.7, "seed": 20261202, "tau": 0.8,
+                                        "magic": 3.0 / 16.0, "fx": 1e-6, "steps": 5000, "stride": 2},
+    # periodic Taylor-Green 512^3, 1000 steps
+    "tgv512_1000steps_f32": {"case": "tgv", "n": 512, "steps": 1000, "stride": 8},
+}
+
+
+def full_size(name: str) -> None:
+    """one FULL workload through the reference's FP32 path, reduced to the
+    strided mid-plane velocity, 256 seeded fluid cells and fluid sums; the
+    core materials are kept as a CRC-32 and per-material counts"""
+    import zlib
+
+    from tests.golden_runs import core_fluid_mask, reduce_state
+
+    P = FULL[name]
+    path = os.path.join(RUN, name + ".bin")
+    extra = {}
+    if P["case"] == "tgv":
+        args = ["tgv", str(P["n"]), str(P["steps"]), "0.04", "0.56", path]
+    else:
+        centres = cases.sphere_pack(P["extent"], P["radius"], P["porosity"], P["seed"])
+        sph = os.path.join(RUN, "spheres_full.txt")
+        np.savetxt(sph, centres, fmt="%.17g")
+        args = ["porous", *map(str, P["extent"]), sph, str(P["radius"]), str(P["tau"]),
+                repr(P["magic"]), repr(P["fx"]), str(P["steps"]), path]
+        extra = {"nspheres": len(centres)}
+    run("olb_ref_f", args, threads=8)
+    meta, arrays = read_dump(path)
+    meta = dict(meta, stride=P["stride"], **extra)
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    mats = np.asarray(arrays["materials"]).astype(np.uint8)
+    core = np.ascontiguousarray(mats.reshape(shape)[p:-p, p:-p, p:-p])
+    fluid = core_fluid_mask(meta, mats)
+    sample = np.sort(np.random.default_rng(64).choice(np.flatnonzero(fluid), 256, replace=False))
+    n = mats.size
+    up, fs, sums = reduce_state(arrays["f1"].reshape(19, n), meta, fluid, sample)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), meta=np.array(json.dumps(meta)),
+                        core_materials_crc32=np.int64(zlib.crc32(core.tobytes())),
+                        material_counts=np.bincount(core.reshape(-1), minlength=4).astype(np.int64),
+                        u_plane=up.astype(np.float32), sample_cells=sample.astype(np.int64),
+                        f1_sample=fs.astype(np.float32), fluid_sums=sums)
+    print("wrote", name, sums)
+    del arrays
+    os.remove(path)
+
+
 def main() -> None:
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            full_size(name)
+        return
     cavity_reduced("olb_ref_f", "f32")
     for tag, binary in (("f32", "olb_ref_f"), ("f64", "olb_ref_d")):
         path = os.path.join(RUN, f"cavity_{tag}.bin")

@@ -1,4 +1,8 @@
-"""Reader of the binary dumps written by oracle/ref_driver.cpp."""
+"""Reader of the binary dumps written by oracle/ref_driver.cpp.
+
+Arrays are memory-mapped (read-only), so dumps of 512^3 blocks (10 GB per
+population array) can be reduced in chunks without loading them whole.
+"""
 from __future__ import annotations
 
 import numpy as np
@@ -6,30 +10,29 @@ import numpy as np
 _DT = {"f4": np.float32, "f8": np.float64, "i4": np.int32, "i8": np.int64}
 
 
-def read_dump(path: str):
-    with open(path, "rb") as fh:
-        data = fh.read()
-    nl = data.index(b"\n")
-    head = data[:nl].decode().split()
-    assert head[0] == "OLBDUMP" and head[1] == "v1", head[:2]
-    meta = {}
-    for kv in head[2:]:
-        k, v = kv.split("=", 1)
+def _value(v: str):
+    for conv in (int, float):
         try:
-            meta[k] = int(v)
+            return conv(v)
         except ValueError:
-            try:
-                meta[k] = float(v)
-            except ValueError:
-                meta[k] = v
-    pos = nl + 1
+            pass
+    return v
+
+
+def read_dump(path: str):
     arrays = {}
-    while pos < len(data):
-        nl = data.index(b"\n", pos)
-        _, name, dt, count = data[pos:nl].decode().split()
-        count = int(count)
-        dtype = np.dtype(_DT[dt])
-        start = nl + 1
-        arrays[name] = np.frombuffer(data[start:start + count * dtype.itemsize], dtype=dtype).copy()
-        pos = start + count * dtype.itemsize
+    with open(path, "rb") as fh:
+        head = fh.readline().decode().split()
+        assert head[0] == "OLBDUMP" and head[1] == "v1", head[:2]
+        meta = {k: _value(v) for k, v in (kv.split("=", 1) for kv in head[2:])}
+        while True:
+            line = fh.readline()
+            if not line:
+                break
+            _, name, dt, count = line.decode().split()
+            count = int(count)
+            dtype = np.dtype(_DT[dt])
+            start = fh.tell()
+            arrays[name] = np.memmap(path, dtype=dtype, mode="r", offset=start, shape=(count,))
+            fh.seek(start + count * dtype.itemsize)
     return meta, arrays

@@ -149,3 +149,52 @@ def cavity_reduced(lat, meta, d):
     uf = u.reshape(-1, 3)[fluid]
     sums = np.array([(rho[fluid] - 1).sum(), *uf.sum(0), (uf ** 2).sum()])
     return mid, f[:, d["sample_cells"]], sums
+
+
+def plane_cells(meta: dict) -> np.ndarray:
+    """CellIDs of the mid-plane z = nz/2 over the core, every `stride`-th
+    cell in x and y (CellID = ((x+p) Py + (y+p)) Pz + (z+p))"""
+    p, s = meta["padding"], meta["stride"]
+    py, pz = meta["ny"] + 2 * p, meta["nz"] + 2 * p
+    xs = np.arange(0, meta["nx"], s) + p
+    ys = np.arange(0, meta["ny"], s) + p
+    zc = p + meta["nz"] // 2
+    return ((xs[:, None] * py + ys[None, :]) * pz + zc).reshape(-1)
+
+
+def core_fluid_mask(meta: dict, materials: np.ndarray) -> np.ndarray:
+    """flat mask of the core cells of material 1 (the padding of a periodic
+    block holds copies that the reference also marks as fluid)"""
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    m = np.zeros(shape, dtype=bool)
+    m[p:-p, p:-p, p:-p] = materials.reshape(shape)[p:-p, p:-p, p:-p] == 1
+    return m.reshape(-1)
+
+
+def _rho_u(f: np.ndarray):
+    from tests.helpers import C
+
+    f = np.asarray(f, dtype=np.float64)
+    rho = f.sum(0) + 1.0
+    return rho, (C.T @ f) / rho
+
+
+def reduce_state(f, meta: dict, fluid: np.ndarray, sample: np.ndarray, chunk: int = 1 << 22):
+    """what a full-size fixture keeps of populations f (19 x ncells, shifted
+    by -t_i, any array-like): the velocity on the strided mid-plane, the
+    populations of the sample cells, and FP64 sums over the core fluid
+    [sum(rho-1), sum ux, sum uy, sum uz, sum u^2], formed in chunks of cells"""
+    plane = plane_cells(meta)
+    _, up = _rho_u(f[:, plane])
+    nxs = len(range(0, meta["nx"], meta["stride"]))
+    up = up.T.reshape(nxs, -1, 3)
+    sums = np.zeros(5)
+    n = f.shape[1]
+    for a in range(0, n, chunk):
+        sel = np.flatnonzero(fluid[a:a + chunk]) + a
+        if sel.size == 0:
+            continue
+        rho, u = _rho_u(f[:, a:a + chunk][:, sel - a])
+        sums += [np.sum(rho - 1.0), *u.sum(1), np.sum(u * u)]
+    return up, np.asarray(f[:, sample], dtype=np.float64), sums

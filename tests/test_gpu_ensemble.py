@@ -143,9 +143,9 @@ def test_autotuners_keep_results(tmp_path, monkeypatch):
     monkeypatch.setenv("OLBGFX_TUNE_CACHE", str(tmp_path / "tune.json"))
     _, a = _members("f32", sizes=[10, 12])
     _, b = _members("f32", sizes=[10, 12])
-    res = autotune.tune_lattice(a[0], steps=2, blocks=(64, 256), waves=(0, 3))
-    assert res["block"] in (64, 256) and len(res["trials"]) == 4
-    b[0].collide_and_stream(4 * (1 + 2))  # four trials of one untimed + two timed steps
+    res = autotune.tune_lattice(a[0], steps=2, blocks=(64, 256), waves=(0, 3), orders=(-1, 2))
+    assert res["block"] in (64, 256) and res["order"] in (-1, 2) and len(res["trials"]) == 5
+    b[0].collide_and_stream(5 * (1 + 2))  # five trials of one untimed + two timed steps
     np.testing.assert_array_equal(a[0].get_populations(), b[0].get_populations())
     ens = Ensemble(a)
     r2 = autotune.tune_ensemble(ens, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2),

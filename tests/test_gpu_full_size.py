@@ -1,0 +1,62 @@
+"""The full-size workloads on the GPU against reductions of the reference's
+own FP32 CPU run (tests/golden/make_fixtures.py, FULL): mid-plane velocity,
+the populations of 256 seeded fluid cells and FP64 sums over the fluid."""
+import numpy as np
+import pytest
+
+from tests.golden_runs import core_fluid_mask, dynamics_for, load, reduce_state
+from tests.helpers import rel_err_unshifted
+
+FULL = ["tgv256_500steps_f32", "porous384x128x128_5000steps_f32", "tgv512_1000steps_f32"]
+
+
+def _case(meta):
+    from openlb_amd import cases
+
+    if meta["case"] == "tgv":
+        return cases.tgv(meta["nx"], u0=0.04, tau=0.56, dtype=np.float32)
+    return cases.porous_channel((meta["nx"], meta["ny"], meta["nz"]), radius=6.0, porosity=0.7,
+                                seed=20261202, force=1e-6, tau=0.8, magic=3.0 / 16.0,
+                                dtype=np.float32)
+
+
+def _have(name):
+    import os
+
+    from tests.golden_runs import GOLDEN
+
+    return os.path.exists(os.path.join(GOLDEN, name + ".npz"))
+
+
+# FP32 tolerances against the reference's FP32 run (different summation
+# order and FMA contraction, growing with the step count)
+TOL = {
+    "tgv256_500steps_f32": {"u": 2e-6, "f": 2e-5, "sums": 2e-5},
+    "porous384x128x128_5000steps_f32": {"u": 2e-8, "f": 2e-5, "sums": 2e-5},
+    "tgv512_1000steps_f32": {"u": 2e-6, "f": 2e-5, "sums": 2e-5},
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", [n for n in FULL if _have(n)])
+def test_full_size_fp32_matches_reference(name):
+    from openlb_amd import Lattice
+
+    meta, d = load(name)
+    case = _case(meta)
+    lat = Lattice(case.extent, case.padding, case.periodic, "f32")
+    case.apply(lat)
+    for mat, dyn in dynamics_for(meta, "f32").items():
+        assert case.dynamics[mat].kind == dyn.kind
+    lat.collide_and_stream(meta["steps"])
+    fluid = core_fluid_mask(meta, case.materials)
+    f = lat.get_populations()
+    up, fs, sums = reduce_state(f, meta, fluid, d["sample_cells"])
+    del f
+    tol = TOL[name]
+    du = float(np.max(np.abs(up - d["u_plane"])))
+    ef = rel_err_unshifted(fs, d["f1_sample"])
+    print(f"{name}: max |du| plane {du:.3e}, sample rel {ef:.3e}, sums {sums} ref {d['fluid_sums']}")
+    assert du <= tol["u"]
+    assert ef <= tol["f"]
+    np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=tol["sums"])

@@ -148,6 +148,33 @@ def test_launch_shapes_give_identical_results(block):
     np.testing.assert_array_equal(alt.get_populations(), ref.get_populations())
 
 
+@pytest.mark.parametrize("order,block", [(0, 64), (2, 64), (4, 128), (6, 256), (12, 64)])
+def test_block_orders_give_identical_results(order, block):
+    """every block order visits each cell once: the masked cavity with
+    statistics on gives the same populations and statistics"""
+    c = cases.box_cavity(20, 0.1, 100.0, dtype=np.float32)
+    out = []
+    for o in (-1, order):
+        lat = Lattice(c.extent, c.padding, c.periodic, "f32")
+        lat.set_launch(block)
+        lat.set_block_order(o)
+        c.apply(lat)
+        lat.set_statistics(True)
+        lat.collide_and_stream(7)
+        out.append((lat.get_populations(), lat.get_statistics()))
+    np.testing.assert_array_equal(out[1][0], out[0][0])
+    assert out[1][1]["nCells"] == out[0][1]["nCells"]
+    assert out[1][1]["maxU"] == out[0][1]["maxU"]
+    assert out[1][1]["averageRho"] == pytest.approx(out[0][1]["averageRho"], rel=1e-14)
+
+
+def test_block_order_outside_range_is_rejected():
+    lat = Lattice((8, 8, 8), 1, (True,) * 3, "f32")
+    for bad in (-2, 13):
+        with pytest.raises(OlbgfxError):
+            lat.set_block_order(bad)
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_storage_rounding_bit_exact(storage):
     """the device conversion to FP16 / BF16 rounds exactly as the oracle
