@@ -47,6 +47,7 @@ def parse_args():
     ap.add_argument("--acc-steps", type=int, default=1000)
     ap.add_argument("--no-porous", action="store_true")
     ap.add_argument("--porous-steps", type=int, default=500)
+    ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     return ap.parse_args()
@@ -124,7 +125,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     tuned = None
     if tune:
         tuned = autotune.tune_ensemble(ens, steps=3, stream=handle, use_cache=False)
-        tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "group_bytes", "steps_per_group", "ticket", "ms")}
+        tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "order", "group_bytes", "steps_per_group", "ticket", "ms")}
     ens.step(warmup, handle)
     steps = max(10, steps // 10 * 10)
     ms_persistent = ens.time("persistent", steps, handle)
@@ -203,9 +204,9 @@ def accuracy(torch, size, steps):
 
 
 def time_porous(torch, steps, storages=("f32", "f16")):
-    """config 4: 384x128x128 periodic sphere pack (r = 6, porosity ~0.7),
+    """porous channel: 384x128x128 periodic sphere pack (r = 6, porosity ~0.7),
     ForcedTRT (Lambda = 3/16) with g_x = 1e-6, BounceBack solids"""
-    from openlb_amd import Lattice, cases
+    from openlb_amd import Lattice, autotune, cases
 
     res = {}
     case = cases.porous_channel((384, 128, 128), radius=6.0, porosity=0.7, seed=20261202, force=1e-6,
@@ -215,15 +216,69 @@ def time_porous(torch, steps, storages=("f32", "f16")):
     for st in storages:
         lat = Lattice(case.extent, case.padding, case.periodic, st)
         case.apply(lat)
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
         lat.collide_and_stream(10, handle)
         ms = lat.time_steps(steps, handle)
         m = lat.reduce_moments(handle)
         res[st] = {"glups": cells * steps / (ms * 1e-3) / 1e9, "ms_per_step": ms / steps,
-                   "mean_ux": m["momentum"][0] / m["nCells"]}
+                   "mean_ux": m["momentum"][0] / m["nCells"],
+                   "launch": {k: tuned[k] for k in ("block", "waves", "order")}}
         lat.close()
     res["porosity"] = case.info["porosity"]
     res["spheres"] = int(len(case.info["spheres"]))
     return res
+
+
+def time_slab(torch, steps, storage="f32", shape=(64, 512, 512)):
+    """one GPU's share of a 512^3 block cut into eight x-slabs: 64x512x512 BGK,
+    periodic along y and z, its two x faces fed through halo planes (wired to
+    each other here, where a multi-GPU run would receive the neighbours'
+    planes); the step with pack/unpack, and the four halo kernels alone"""
+    from openlb_amd import BGK, Dynamics, Lattice, autotune, cases
+
+    p = 3
+    mats = np.zeros(tuple(n + 2 * p for n in shape), dtype=np.uint8)
+    mats[p:-p, p:-p, p:-p] = 1
+    case = cases.Case(name="slab", extent=shape, padding=p, periodic=(False, True, True), materials=mats,
+                      dynamics={1: Dynamics(BGK, omega=1 / 0.56)})
+    lat = Lattice(case.extent, case.padding, case.periodic, storage)
+    case.apply(lat)
+    stream = torch.cuda.current_stream()
+    handle = stream.cuda_stream
+    tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+    nbytes = lat.halo_bytes(0)
+    lo, hi = lat.halo_buffer(0), lat.halo_buffer(0)
+
+    def exchange():
+        lat.halo_pack(0, 1, hi.ptr, handle)
+        lat.halo_pack(0, 0, lo.ptr, handle)
+        lat.halo_unpack(0, 0, hi.ptr, handle)
+        lat.halo_unpack(0, 1, lo.ptr, handle)
+
+    def timed(fn, k):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(k):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / k
+
+    def step():
+        lat.collide_and_stream(1, handle)
+        exchange()
+
+    for _ in range(5):
+        step()
+    ms_step = timed(step, steps)
+    ms_halo = timed(exchange, steps)
+    lo.close()
+    hi.close()
+    lat.close()
+    cells = int(np.prod(shape))
+    return {"shape": list(shape), "storage": storage, "glups": cells / (ms_step * 1e-3) / 1e9,
+            "ms_per_step": ms_step, "ms_halo_kernels": ms_halo, "halo_bytes_per_face": nbytes,
+            "launch": {k: tuned[k] for k in ("block", "waves", "order")}}
 
 
 def copy_bandwidth(torch, nbytes=4 << 30, reps=10):
@@ -320,6 +375,11 @@ def main():
     if porous:
         print(f"# porous: f32 {porous['f32']['glups']:.2f} GLUPS, f16 {porous['f16']['glups']:.2f} GLUPS",
               file=sys.stderr)
+    slab = None if args.no_slab else time_slab(torch, args.steps)
+    if slab:
+        print(f"# slab {slab['shape']}: {slab['glups']:.2f} GLUPS with halo planes, "
+              f"halo kernels {slab['ms_halo_kernels'] * 1e3:.1f} us of {slab['ms_per_step'] * 1e3:.1f} us",
+              file=sys.stderr)
     acc = None if args.no_accuracy else accuracy(torch, args.acc_size, args.acc_steps)
     if acc:
         print("# accuracy vs f64: " + ", ".join(f"{k} L2 {acc[k]['velocity_l2_vs_f64']:.2e}"
@@ -367,6 +427,7 @@ def main():
         "ensemble": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in ens.items()},
         "porous": porous,
+        "slab": slab,
         "accuracy": acc,
         "roofline": roofline,
         "cpu_baseline": base,

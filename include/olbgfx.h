@@ -193,6 +193,10 @@ olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* ensemble, uint32_t blo
 olbgfx_status olbgfx_ensemble_set_schedule(olbgfx_ensemble* ensemble, uint64_t group_bytes,
                                            uint32_t steps_per_group, int32_t ticket);
 
+/* block order of the persistent kernel, as olbgfx_set_block_order; drops a
+ * captured graph.  Used by the autotuner. */
+olbgfx_status olbgfx_ensemble_set_block_order(olbgfx_ensemble* ensemble, int32_t xcd_run_log2);
+
 /* advance every member by `steps` steps, one persistent launch per step */
 olbgfx_status olbgfx_ensemble_step(olbgfx_ensemble* ensemble, uint32_t steps, void* stream);
 
@@ -217,6 +221,95 @@ olbgfx_status olbgfx_ensemble_cells(const olbgfx_ensemble* ensemble, uint64_t* c
 olbgfx_status olbgfx_time_steps(olbgfx_lattice* lattice, uint32_t steps, void* stream, float* ms);
 olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* ensemble, int32_t mode, uint32_t steps,
                                    void* stream, float* ms);
+
+/* ---- halo planes of a block split over several GPUs -------------------------
+ * For a non-periodic face (axis 0..2, side 0 = low, 1 = high) of a block whose
+ * neighbour block lives elsewhere.  After a step, the five populations that
+ * left the core through the face sit in the ghost plane beyond it;
+ * olbgfx_halo_pack gathers them into a device buffer (population-major, the
+ * plane's cells in CellID order, raw storage words).  The neighbour passes
+ * that buffer to olbgfx_halo_unpack for its facing side, which writes them as
+ * the incoming populations of its boundary plane, so the next step sees what
+ * one undivided block would.  Both sides must have the same cross-section.
+ * Stands in for the overlap copy between blocks of
+ * ConcreteBlockCommunicator (communication/blockCommunicationNeighborhood.hh
+ * :67-107, core/platform/gpu/hip/communicator.hh:529-585), reduced to the
+ * populations that cross the face.  Enqueued on `stream`. */
+olbgfx_status olbgfx_halo_bytes(const olbgfx_lattice* lattice, int32_t axis, uint64_t* bytes);
+/* a device buffer of olbgfx_halo_bytes for faces normal to `axis`, owned by
+ * the library until olbgfx_halo_buffer_destroy (a caller may pass its own
+ * device memory to pack/unpack instead, e.g. an RCCL receive buffer) */
+olbgfx_status olbgfx_halo_buffer_create(const olbgfx_lattice* lattice, int32_t axis, void** device_buffer);
+olbgfx_status olbgfx_halo_buffer_destroy(void* device_buffer);
+olbgfx_status olbgfx_halo_pack(olbgfx_lattice* lattice, int32_t axis, int32_t side,
+                               void* device_buffer, void* stream);
+olbgfx_status olbgfx_halo_unpack(olbgfx_lattice* lattice, int32_t axis, int32_t side,
+                                 const void* device_buffer, void* stream);
+
+/* ---- autotuner ---------------------------------------------------------------
+ * Times candidate launch configurations on the lattice (ensemble) itself
+ * with HIP events on `stream`, applies the fastest, and leaves the simulation
+ * state as it found it: populations, offsets and statistics are saved in
+ * device memory before the trials and put back after them (this needs a
+ * second copy of the populations; ERR_NOMEM if it does not fit).
+ *
+ * Lattices: first every (block_threads x waves_per_simd) pair with the
+ * default block order, then every block order with the three fastest pairs.
+ * Ensembles: first every (block_threads x waves_per_simd x tiles_per_block)
+ * with one group, then every block order with the three fastest, then every
+ * (group_bytes x steps_per_group x ticket) with the best launch.  A list with n == 0 takes the built-in candidates; a null
+ * `space` takes them all.  Each trial runs one untimed step (steps_per_group
+ * for schedules) and then `steps` timed steps (ensembles: at least 8).
+ *
+ * `cache_path` (may be null) names a text file of earlier choices keyed by
+ * extent, padding, periodicity, storage and the dynamics mix (ensembles: by
+ * all members); a hit is applied without trials and reported with cached = 1.
+ * No reference counterpart: OpenLB fixes its launch shapes
+ * (core/platform/gpu/hip/operator.hh:426-433). */
+#define OLBGFX_TUNE_CHOICES 8
+#define OLBGFX_TUNE_TRIALS 128
+
+typedef struct olbgfx_tune_space {
+  uint32_t n_block_threads;
+  uint32_t block_threads[OLBGFX_TUNE_CHOICES];
+  uint32_t n_waves;
+  uint32_t waves_per_simd[OLBGFX_TUNE_CHOICES];
+  uint32_t n_orders;
+  int32_t block_order[OLBGFX_TUNE_CHOICES];
+  uint32_t n_tiles;
+  uint32_t tiles_per_block[OLBGFX_TUNE_CHOICES];
+  uint32_t n_group_bytes;
+  uint64_t group_bytes[OLBGFX_TUNE_CHOICES];
+  uint32_t n_steps_per_group;
+  uint32_t steps_per_group[OLBGFX_TUNE_CHOICES];
+  uint32_t n_tickets;
+  int32_t ticket[2];
+} olbgfx_tune_space;
+
+typedef struct olbgfx_tune_trial {
+  uint32_t block_threads;
+  uint32_t waves_per_simd;
+  int32_t block_order;        /* lattices */
+  uint32_t tiles_per_block;   /* ensembles */
+  uint64_t group_bytes;       /* ensembles */
+  uint32_t steps_per_group;   /* ensembles */
+  int32_t ticket;             /* ensembles */
+  double ms_per_step;
+} olbgfx_tune_trial;
+
+typedef struct olbgfx_tune_result {
+  olbgfx_tune_trial best;
+  int32_t cached;
+  uint32_t ntrials;
+  olbgfx_tune_trial trials[OLBGFX_TUNE_TRIALS];
+} olbgfx_tune_result;
+
+olbgfx_status olbgfx_autotune_lattice(olbgfx_lattice* lattice, uint32_t steps,
+                                      const olbgfx_tune_space* space, const char* cache_path,
+                                      void* stream, olbgfx_tune_result* out);
+olbgfx_status olbgfx_autotune_ensemble(olbgfx_ensemble* ensemble, uint32_t steps,
+                                       const olbgfx_tune_space* space, const char* cache_path,
+                                       void* stream, olbgfx_tune_result* out);
 
 /* block until all work of this lattice is done */
 olbgfx_status olbgfx_synchronize(olbgfx_lattice* lattice);

@@ -8,6 +8,12 @@ block-lattice calls of OpenLB's platform interface: dynamics per cell,
 
 There is no CPU fallback: if the library is missing or cannot be loaded, the
 import of :class:`Lattice` fails with an error that says so.
+
+A process that also uses PyTorch on the GPU must import torch first: torch's
+wheel carries its own HIP runtime under the same soname, and the library then
+binds to it, so both share one runtime (streams and events interoperate).
+Loaded the other way round, two HIP runtimes end up in the process and
+torch's finds no GPU.
 """
 from __future__ import annotations
 
@@ -109,12 +115,20 @@ def library() -> ctypes.CDLL:
         "olbgfx_ensemble_set_launch": [vp, u32, u32, u32],
         "olbgfx_ensemble_step": [vp, u32, vp],
         "olbgfx_ensemble_set_schedule": [vp, ctypes.c_uint64, u32, i32],
+        "olbgfx_ensemble_set_block_order": [vp, i32],
         "olbgfx_ensemble_step_separately": [vp, u32, vp],
         "olbgfx_ensemble_capture": [vp, u32],
         "olbgfx_ensemble_launch_graph": [vp, u32, vp],
         "olbgfx_ensemble_cells": [vp, ctypes.POINTER(ctypes.c_uint64)],
         "olbgfx_time_steps": [vp, u32, vp, ctypes.POINTER(ctypes.c_float)],
         "olbgfx_ensemble_time": [vp, i32, u32, vp, ctypes.POINTER(ctypes.c_float)],
+        "olbgfx_autotune_lattice": [vp, u32, vp, ctypes.c_char_p, vp, vp],
+        "olbgfx_halo_bytes": [vp, i32, ctypes.POINTER(ctypes.c_uint64)],
+        "olbgfx_halo_buffer_create": [vp, i32, ctypes.POINTER(vp)],
+        "olbgfx_halo_buffer_destroy": [vp],
+        "olbgfx_halo_pack": [vp, i32, i32, vp, vp],
+        "olbgfx_halo_unpack": [vp, i32, i32, vp, vp],
+        "olbgfx_autotune_ensemble": [vp, u32, vp, ctypes.c_char_p, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -290,6 +304,29 @@ class Lattice:
         of 2^k blocks (results do not change)"""
         _check(library().olbgfx_set_block_order(self._h, int(xcd_run_log2)))
 
+    # -- halo planes of a block split over several GPUs ------------------------------
+
+    def halo_bytes(self, axis: int) -> int:
+        n = ctypes.c_uint64()
+        _check(library().olbgfx_halo_bytes(self._h, int(axis), ctypes.byref(n)))
+        return n.value
+
+    def halo_buffer(self, axis: int) -> "HaloBuffer":
+        """a library-owned device buffer for one packed face normal to axis"""
+        p = ctypes.c_void_p()
+        _check(library().olbgfx_halo_buffer_create(self._h, int(axis), ctypes.byref(p)))
+        return HaloBuffer(p.value, self.halo_bytes(axis))
+
+    def halo_pack(self, axis: int, side: int, device_ptr: int, stream=None) -> None:
+        """gather the populations that left through face (axis, side) into the
+        device buffer at device_ptr"""
+        _check(library().olbgfx_halo_pack(self._h, int(axis), int(side), ctypes.c_void_p(device_ptr), stream))
+
+    def halo_unpack(self, axis: int, side: int, device_ptr: int, stream=None) -> None:
+        """write a neighbour's packed plane as the incoming populations of
+        face (axis, side)"""
+        _check(library().olbgfx_halo_unpack(self._h, int(axis), int(side), ctypes.c_void_p(device_ptr), stream))
+
     # -- statistics -----------------------------------------------------------------
 
     def set_statistics(self, on: bool) -> None:
@@ -328,6 +365,25 @@ class Lattice:
         return rho, u.reshape(self.ncells, 3)
 
 
+class HaloBuffer:
+    """device memory for one packed halo plane (olbgfx_halo_buffer_create)"""
+
+    def __init__(self, ptr: int, nbytes: int):
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def close(self) -> None:
+        if self.ptr:
+            library().olbgfx_halo_buffer_destroy(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Ensemble:
     """A batch of independent lattices advanced by one persistent kernel per
     step (optionally replayed from a captured hipGraph)."""
@@ -356,6 +412,10 @@ class Ensemble:
     def set_launch(self, block_threads: int = 256, waves_per_simd: int = 0, tiles_per_block: int = 1) -> None:
         _check(library().olbgfx_ensemble_set_launch(self._h, int(block_threads), int(waves_per_simd),
                                                      int(tiles_per_block)))
+
+    def set_block_order(self, xcd_run_log2: int = -1) -> None:
+        """block order of the persistent kernel (see Lattice.set_block_order)"""
+        _check(library().olbgfx_ensemble_set_block_order(self._h, int(xcd_run_log2)))
 
     def set_schedule(self, group_bytes: int = 0, steps_per_group: int = 1, ticket: bool = False) -> None:
         """pack members into groups of at most group_bytes of populations (0 =

@@ -1,134 +1,119 @@
-"""Launch-shape autotuner for the step kernels.
+"""Python face of the library's launch autotuner (olbgfx_autotune_lattice,
+olbgfx_autotune_ensemble).
 
-For a lattice it sweeps threads per block and an occupancy cap (waves per
-SIMD, enforced by an LDS reservation); for an ensemble also the number of
-tiles each persistent block walks.  Every candidate is timed with HIP events
-(olbgfx_time_steps / olbgfx_ensemble_time) after one untimed step, and the
-fastest is applied.  Results are cached in a JSON file keyed by shape,
-storage and dynamics mix, so a later run with the same key skips the sweep.
-
-The trial steps advance the lattice: tune during warm-up.
+For a lattice the library sweeps threads per block and an occupancy cap
+(waves per SIMD, enforced by an LDS reservation), then the order in which
+blocks visit the cells; for an ensemble the tiles each persistent block walks
+and the group schedule.  Every candidate is timed with HIP events on the
+lattice itself, the fastest is applied, and the populations, offsets and
+statistics are put back as they were, so tuning does not advance the
+simulation.  Choices are cached in a text file keyed by shape, storage and
+dynamics mix ($OLBGFX_TUNE_CACHE, default ~/.cache/olbgfx/autotune.txt).
 """
 from __future__ import annotations
 
-import hashlib
-import json
+import ctypes
 import os
 
-BLOCKS = (64, 128, 256)
-ORDERS = (-1, 0, 2, 4, 6)  # block orders: contiguous per XCD, runs of 2^k blocks
-WAVES = (0, 2, 3, 4, 5, 6)
-TILES = (1, 2, 4, 8)
+from . import _check, library
+
+CHOICES = 8
+TRIALS = 128
 
 
-def _cache_path() -> str:
+class TuneSpace(ctypes.Structure):
+    _fields_ = [
+        ("n_block_threads", ctypes.c_uint32), ("block_threads", ctypes.c_uint32 * CHOICES),
+        ("n_waves", ctypes.c_uint32), ("waves_per_simd", ctypes.c_uint32 * CHOICES),
+        ("n_orders", ctypes.c_uint32), ("block_order", ctypes.c_int32 * CHOICES),
+        ("n_tiles", ctypes.c_uint32), ("tiles_per_block", ctypes.c_uint32 * CHOICES),
+        ("n_group_bytes", ctypes.c_uint32), ("group_bytes", ctypes.c_uint64 * CHOICES),
+        ("n_steps_per_group", ctypes.c_uint32), ("steps_per_group", ctypes.c_uint32 * CHOICES),
+        ("n_tickets", ctypes.c_uint32), ("ticket", ctypes.c_int32 * 2),
+    ]
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [
+        ("block_threads", ctypes.c_uint32), ("waves_per_simd", ctypes.c_uint32),
+        ("block_order", ctypes.c_int32), ("tiles_per_block", ctypes.c_uint32),
+        ("group_bytes", ctypes.c_uint64), ("steps_per_group", ctypes.c_uint32),
+        ("ticket", ctypes.c_int32), ("ms_per_step", ctypes.c_double),
+    ]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best", TuneTrial), ("cached", ctypes.c_int32), ("ntrials", ctypes.c_uint32),
+                ("trials", TuneTrial * TRIALS)]
+
+
+def cache_path() -> str:
     return os.environ.get("OLBGFX_TUNE_CACHE",
-                          os.path.join(os.path.expanduser("~"), ".cache", "olbgfx", "autotune.json"))
+                          os.path.join(os.path.expanduser("~"), ".cache", "olbgfx", "autotune.txt"))
 
 
-def _load() -> dict:
-    try:
-        with open(_cache_path()) as fh:
-            return json.load(fh)
-    except (OSError, ValueError):
-        return {}
-
-
-def _store(key: str, value: dict) -> None:
-    path = _cache_path()
-    data = _load()
-    data[key] = value
-    try:
-        os.makedirs(os.path.dirname(path), exist_ok=True)
-        with open(path + ".tmp", "w") as fh:
-            json.dump(data, fh, indent=1, sort_keys=True)
-        os.replace(path + ".tmp", path)
-    except OSError:
-        pass
-
-
-def lattice_key(lat, mask_digest: str = "") -> str:
-    return f"lattice:{lat.extent}:{lat.padding}:{lat.periodic}:{lat.storage}:{mask_digest}"
-
-
-def tune_lattice(lat, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
-                 blocks=BLOCKS, waves=WAVES, orders=ORDERS) -> dict:
-    """pick (block_threads, waves_per_simd), then the block order, for one
-    lattice and apply them"""
-    key = key or lattice_key(lat)
-    if use_cache:
-        hit = _load().get(key)
-        if hit:
-            lat.set_launch(hit["block"], hit["waves"])
-            lat.set_block_order(hit.get("order", -1))
-            return dict(hit, cached=True)
-    trials = []
-    lat.set_block_order(-1)
-    for b in blocks:
-        for w in waves:
-            lat.set_launch(b, w)
-            lat.collide_and_stream(1, stream)
-            ms = lat.time_steps(steps, stream) / steps
-            trials.append({"block": b, "waves": w, "order": -1, "ms": ms})
-    best = min(trials, key=lambda t: t["ms"])
-    lat.set_launch(best["block"], best["waves"])
-    for o in orders:
-        if o == -1:
+def _space(**lists) -> TuneSpace:
+    sp = TuneSpace()
+    for name, count, values in (("block_threads", "n_block_threads", lists.get("blocks")),
+                                ("waves_per_simd", "n_waves", lists.get("waves")),
+                                ("block_order", "n_orders", lists.get("orders")),
+                                ("tiles_per_block", "n_tiles", lists.get("tiles")),
+                                ("group_bytes", "n_group_bytes", lists.get("group_bytes")),
+                                ("steps_per_group", "n_steps_per_group", lists.get("steps_per_group")),
+                                ("ticket", "n_tickets", lists.get("tickets"))):
+        if values is None:
             continue
-        lat.set_block_order(o)
-        lat.collide_and_stream(1, stream)
-        ms = lat.time_steps(steps, stream) / steps
-        trials.append({"block": best["block"], "waves": best["waves"], "order": o, "ms": ms})
-    best = min(trials, key=lambda t: t["ms"])
-    lat.set_launch(best["block"], best["waves"])
-    lat.set_block_order(best["order"])
-    pick = {k: best[k] for k in ("block", "waves", "order", "ms")}
-    _store(key, pick)
-    return dict(pick, trials=trials)
+        arr = getattr(sp, name)
+        if len(values) > len(arr):
+            raise ValueError(f"at most {len(arr)} candidates for {name}")
+        for k, v in enumerate(values):
+            arr[k] = int(v)
+        setattr(sp, count, len(values))
+    return sp
 
 
-def ensemble_key(extents, storage: str) -> str:
-    h = hashlib.sha1(repr(sorted(extents)).encode()).hexdigest()[:12]
-    return f"ensemble:{len(extents)}:{h}:{storage}"
+def _trial(t: TuneTrial, ensemble: bool) -> dict:
+    d = {"block": t.block_threads, "waves": t.waves_per_simd, "order": t.block_order, "ms": t.ms_per_step}
+    if ensemble:
+        d.update(tiles=t.tiles_per_block, group_bytes=t.group_bytes, steps_per_group=t.steps_per_group,
+                 ticket=bool(t.ticket))
+    return d
 
 
-def tune_ensemble(ens, steps: int = 4, stream=None, key: str | None = None, use_cache: bool = True,
-                  blocks=(128, 256), waves=(0, 3, 4, 5), tiles=(1, 2, 4),
-                  group_bytes=(0, 96 << 20, 160 << 20), steps_per_group=(1, 4, 8),
-                  tickets=(False, True)) -> dict:
-    """pick (block_threads, waves_per_simd, tiles_per_block), then the group
-    schedule (group_bytes, steps_per_group), for an ensemble"""
-    key = key or ensemble_key([m.extent for m in ens.members], ens.members[0].storage)
-    if use_cache:
-        hit = _load().get(key)
-        if hit:
-            ens.set_launch(hit["block"], hit["waves"], hit["tiles"])
-            ens.set_schedule(hit["group_bytes"], hit["steps_per_group"], hit.get("ticket", False))
-            return dict(hit, cached=True)
-    trials = []
-    span = max(8, steps)
-    ens.set_schedule(0, 1, False)
-    for b in blocks:
-        for w in waves:
-            for t in tiles:
-                ens.set_launch(b, w, t)
-                ens.step(1, stream)
-                ms = ens.time("persistent", span, stream) / span
-                trials.append({"block": b, "waves": w, "tiles": t, "group_bytes": 0, "steps_per_group": 1,
-                               "ticket": False, "ms": ms})
-    best = min(trials, key=lambda t: t["ms"])
-    ens.set_launch(best["block"], best["waves"], best["tiles"])
-    for gb in group_bytes:
-        for k in steps_per_group:
-            for tk in tickets:
-                ens.set_schedule(gb, k, tk)
-                ens.step(k, stream)
-                ms = ens.time("persistent", span, stream) / span
-                trials.append({"block": best["block"], "waves": best["waves"], "tiles": best["tiles"],
-                               "group_bytes": gb, "steps_per_group": k, "ticket": tk, "ms": ms})
-    best = min(trials, key=lambda t: t["ms"])
-    ens.set_launch(best["block"], best["waves"], best["tiles"])
-    ens.set_schedule(best["group_bytes"], best["steps_per_group"], best["ticket"])
-    keep = ("block", "waves", "tiles", "group_bytes", "steps_per_group", "ticket", "ms")
-    _store(key, {k: best[k] for k in keep})
-    return dict({k: best[k] for k in keep}, trials=trials)
+def _result(res: TuneResult, ensemble: bool) -> dict:
+    out = _trial(res.best, ensemble)
+    out["cached"] = bool(res.cached)
+    out["trials"] = [_trial(res.trials[k], ensemble) for k in range(res.ntrials)]
+    return out
+
+
+def _cache_arg(use_cache: bool):
+    if not use_cache:
+        return None
+    path = cache_path()
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    return path.encode()
+
+
+def tune_lattice(lat, steps: int = 4, stream=None, use_cache: bool = True,
+                 blocks=None, waves=None, orders=None) -> dict:
+    """pick (block_threads, waves_per_simd), then the block order, for one
+    lattice and apply them; None takes the library's candidates"""
+    res = TuneResult()
+    sp = _space(blocks=blocks, waves=waves, orders=orders)
+    _check(library().olbgfx_autotune_lattice(lat._h, int(steps), ctypes.byref(sp), _cache_arg(use_cache),
+                                             stream, ctypes.byref(res)))
+    return _result(res, False)
+
+
+def tune_ensemble(ens, steps: int = 4, stream=None, use_cache: bool = True, blocks=None, waves=None,
+                  tiles=None, orders=None, group_bytes=None, steps_per_group=None, tickets=None) -> dict:
+    """pick (block_threads, waves_per_simd, tiles_per_block), then the block
+    order, then the group schedule (group_bytes, steps_per_group, ticket), for
+    an ensemble"""
+    res = TuneResult()
+    sp = _space(blocks=blocks, waves=waves, tiles=tiles, orders=orders, group_bytes=group_bytes,
+                steps_per_group=steps_per_group, tickets=tickets)
+    _check(library().olbgfx_autotune_ensemble(ens._h, int(steps), ctypes.byref(sp), _cache_arg(use_cache),
+                                              stream, ctypes.byref(res)))
+    return _result(res, True)

@@ -200,55 +200,63 @@ def sphere_pack(extent, radius: float, porosity: float, seed: int, max_tries: in
     """Non-overlapping spheres (centres in lattice units) placed by random
     sequential addition until the solid fraction reaches 1 - porosity.
     Spheres may cross the periodic faces; overlap is tested with the minimum
-    image convention."""
+    image convention (squared distance summed x, y, z in order)."""
     rng = _SplitMix64(seed)
     ext = np.asarray(extent, dtype=np.float64)
     vol = float(np.prod(ext))
     target = (1.0 - porosity) * vol
     sphere_vol = 4.0 / 3.0 * np.pi * radius ** 3
-    centres: list = []
+    limit = (2.0 * radius) ** 2
+    cap = int(target / sphere_vol) + 2
+    centres = np.empty((cap, 3))
+    n = 0
     tries = 0
-    while len(centres) * sphere_vol < target and tries < max_tries:
+    while n * sphere_vol < target and tries < max_tries:
         tries += 1
         c = np.array([rng.uniform() * ext[a] for a in range(3)])
-        ok = True
-        for d in centres:
-            diff = np.abs(c - d)
+        if n:
+            diff = np.abs(centres[:n] - c)
             diff = np.minimum(diff, ext - diff)
-            if float(diff @ diff) < (2.0 * radius) ** 2:
-                ok = False
-                break
-        if ok:
-            centres.append(c)
-    return np.array(centres).reshape(-1, 3)
+            d2 = diff[:, 0] * diff[:, 0] + diff[:, 1] * diff[:, 1] + diff[:, 2] * diff[:, 2]
+            if np.any(d2 < limit):
+                continue
+        centres[n] = c
+        n += 1
+    return centres[:n].copy()
 
 
-def porous_materials(extent, centres, radius: float, padding: int) -> np.ndarray:
-    """material 1 fluid, 2 solid (cell centre within radius of a sphere centre,
-    periodic images included), 0 in the padding."""
+def porous_materials(extent, centres, radius: float, padding: int, dtype=np.float64) -> np.ndarray:
+    """material 1 fluid, 2 solid, 0 in the padding.  A core cell is solid when
+    it lies inside a sphere or one of its periodic images by the rule of
+    IndicatorSphere3D (functors/analytical/indicator/indicatorF3D.hh:166-178):
+    d2 < r^2 + eps*r, evaluated in the value type `dtype` with the image
+    centre rounded to it, as SuperGeometry::rename(1, 2, sphere) does for
+    each image."""
+    T = np.dtype(dtype).type
     shape = _padded(extent, padding)
     mats = np.zeros(shape, dtype=np.uint8)
-    nx, ny, nz = extent
+    ext = np.asarray(extent)
     if min(extent) < 2 * radius + 2:
         raise ValueError("every extent must exceed the sphere diameter")
-    core = np.ones((nx, ny, nz), dtype=np.uint8)
-    r2 = radius * radius
-    for c in centres:
-        lo = np.floor(c - radius).astype(int)
-        hi = np.ceil(c + radius).astype(int)
-        xs = np.arange(lo[0], hi[0] + 1)
-        ys = np.arange(lo[1], hi[1] + 1)
-        zs = np.arange(lo[2], hi[2] + 1)
-        dx = (xs - c[0]) ** 2
-        dy = (ys - c[1]) ** 2
-        dz = (zs - c[2]) ** 2
-        inside = (dx[:, None, None] + dy[None, :, None] + dz[None, None, :]) <= r2
-        ix = np.mod(xs, nx)
-        iy = np.mod(ys, ny)
-        iz = np.mod(zs, nz)
-        sub = core[np.ix_(ix, iy, iz)]
-        sub[inside] = 2
-        core[np.ix_(ix, iy, iz)] = sub
+    core = np.ones(tuple(extent), dtype=np.uint8)
+    r = T(radius)
+    thr = T(r * r + T(np.finfo(T).eps) * r)
+    for c in np.asarray(centres, dtype=np.float64):
+        for ix in (-1, 0, 1):
+            for iy in (-1, 0, 1):
+                for iz in (-1, 0, 1):
+                    img = c + np.array([ix, iy, iz]) * ext
+                    lo = np.maximum(np.floor(img - radius).astype(int), 0)
+                    hi = np.minimum(np.ceil(img + radius).astype(int), ext - 1)
+                    if np.any(lo > hi):
+                        continue
+                    cf = img.astype(T)
+                    dx = (np.arange(lo[0], hi[0] + 1).astype(T) - cf[0])[:, None, None]
+                    dy = (np.arange(lo[1], hi[1] + 1).astype(T) - cf[1])[None, :, None]
+                    dz = (np.arange(lo[2], hi[2] + 1).astype(T) - cf[2])[None, None, :]
+                    inside = (dx * dx + dy * dy) + dz * dz < thr
+                    sub = core[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
+                    sub[inside] = 2
     p = padding
     mats[p:-p, p:-p, p:-p] = core
     return mats
@@ -261,7 +269,7 @@ def porous_channel(extent=(384, 128, 128), radius: float = 6.0, porosity: float 
     Lambda = magic and Guo forcing (ForcedTRTdynamics) in the fluid, BounceBack
     in the solid, periodic in x, y and z, starting at rest."""
     centres = sphere_pack(extent, radius, porosity, seed)
-    mats = porous_materials(extent, centres, radius, padding)
+    mats = porous_materials(extent, centres, radius, padding, dtype=dtype)
     shape = mats.shape
     omega = 1.0 / tau
     rho = np.ones(shape, dtype=dtype)

@@ -601,6 +601,16 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
 // MODE = k: every core cell has the dynamics of table slot uniform_id, whose
 //           kind is k, so the mask is not read and only that path is compiled.
 
+// logical block of hardware workgroup b (see the step kernel)
+__device__ __forceinline__ uint32_t logical_block(uint32_t b, uint32_t blocks_per_xcd, int32_t run_log2)
+{
+  if (run_log2 < 0) {
+    return (b & 7u) * blocks_per_xcd + (b >> 3);
+  }
+  const uint32_t k = (uint32_t)run_log2, g = b >> 3;
+  return ((g >> k) << (k + 3)) + ((b & 7u) << k) + (g & ((1u << k) - 1u));
+}
+
 template <typename S, int MODE, bool STATS>
 __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename Storage<S>::A> a)
 {
@@ -612,14 +622,7 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
   // xcd_run_log2 = k the XCDs take turns on runs of 2^k consecutive blocks,
   // which keeps the cells in flight on the whole chip close together.  Both
   // maps are bijections of [0, 8 * blocks_per_xcd).
-  const uint32_t b = blockIdx.x;
-  uint32_t lb;
-  if (a.xcd_run_log2 < 0) {
-    lb = (b & 7u) * a.blocks_per_xcd + (b >> 3);
-  } else {
-    const uint32_t k = (uint32_t)a.xcd_run_log2, g = b >> 3;
-    lb = ((g >> k) << (k + 3)) + ((b & 7u) << k) + (g & ((1u << k) - 1u));
-  }
+  const uint32_t lb = logical_block(blockIdx.x, a.blocks_per_xcd, a.xcd_run_log2);
   const uint32_t lin = lb * blockDim.x + threadIdx.x;
   const bool in_grid = lb < a.nblocks;
   const bool active = in_grid && lin < a.ncore;
@@ -671,7 +674,8 @@ struct GroupArgs {
   uint32_t member_end;
   uint32_t tiles_per_block;
   uint32_t nblocks;          // logical blocks that carry tiles
-  uint32_t blocks_per_xcd;
+  uint32_t blocks_per_xcd;   // rounded up to whole runs
+  int32_t xcd_run_log2;      // block order, as in the step kernel
   uint32_t* counter;         // ticket counter of the group (0 between launches)
 };
 
@@ -682,8 +686,7 @@ __global__ void __launch_bounds__(kMaxBlock)
                     const uint32_t* counts)
 {
   using A = typename Storage<S>::A;
-  const uint32_t b = blockIdx.x;
-  const uint32_t lb = (b & 7u) * g.blocks_per_xcd + (b >> 3);
+  const uint32_t lb = logical_block(blockIdx.x, g.blocks_per_xcd, g.xcd_run_log2);
   if (lb < g.nblocks) {
     const uint32_t end = g.tile_begin + g.ntiles;
     const uint32_t t0 = g.tile_begin + lb * g.tiles_per_block;
@@ -1682,6 +1685,7 @@ struct olbgfx_ensemble {
   uint32_t block_threads = 256;
   uint32_t waves_per_simd = 0;
   uint32_t tiles_per_block = 1;
+  int32_t xcd_run_log2 = -1;         // block order (olbgfx_ensemble_set_block_order)
   size_t lds = 0;
   std::vector<uint64_t> versions;
   uint32_t tiled_block = 0;
@@ -1848,6 +1852,11 @@ olbgfx_status ensemble_enqueue(olbgfx_ensemble* E, uint32_t steps, hipStream_t s
                                    (g.ntiles + kMaxEnsembleBlocks - 1) / kMaxEnsembleBlocks);
       g.nblocks = (g.ntiles + g.tiles_per_block - 1) / g.tiles_per_block;
       g.blocks_per_xcd = (g.nblocks + 7) / 8;
+      g.xcd_run_log2 = E->xcd_run_log2;
+      if (g.xcd_run_log2 >= 0) {
+        const uint32_t run = 1u << g.xcd_run_log2;
+        g.blocks_per_xcd = (g.blocks_per_xcd + run - 1) / run * run;
+      }
       g.ticket = E->ticket;
       const dim3 grid(8 * g.blocks_per_xcd), block(E->block_threads);
       const uint32_t nm = (g.member_end - g.member_begin) * Q;
@@ -1968,6 +1977,19 @@ olbgfx_status olbgfx_ensemble_set_launch(olbgfx_ensemble* E, uint32_t block_thre
   E->waves_per_simd = waves_per_simd;
   E->tiles_per_block = tiles_per_block;
   E->lds = lds_for(block_threads, waves_per_simd);
+  drop_graph(E);
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_ensemble_set_block_order(olbgfx_ensemble* E, int32_t xcd_run_log2)
+{
+  if (!E) {
+    return fail(OLBGFX_ERR_INVALID, "null ensemble");
+  }
+  if (xcd_run_log2 < -1 || xcd_run_log2 > 12) {
+    return fail(OLBGFX_ERR_INVALID, "block order must be -1 or a run length 2^k with k in [0, 12]");
+  }
+  E->xcd_run_log2 = xcd_run_log2;
   drop_graph(E);
   return OLBGFX_OK;
 }
@@ -2167,6 +2189,602 @@ olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* E, int32_t mode, uint32_t st
   default:
     return fail(OLBGFX_ERR_INVALID, "mode: 0 persistent, 1 graph, 2 separate launches");
   }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// halo planes of a non-periodic face, for a block split over several GPUs
+//
+// After a step, the populations that left the core through a face sit in the
+// ghost plane beyond it (the periodic images along the other axes included);
+// the neighbouring block needs exactly these as the incoming populations of
+// its own boundary plane, cell for cell.  pack gathers them from the ghost
+// plane, unpack writes them into the boundary plane; for a face whose outward
+// normal is +e_a the populations are those with c_a = +1 on both sides.
+
+namespace {
+
+struct HaloArgs {
+  uint32_t shift[5];      // ring offsets of the five populations
+  uint32_t pops[5];       // their indices
+  int64_t base;           // CellID of plane cell (0, 0)
+  int64_t step_u, step_v; // CellID strides of the two in-plane axes
+  uint32_t ext_v;         // cells along v
+  uint32_t plane;         // cells per plane
+};
+
+template <typename S, bool PACK>
+__global__ void __launch_bounds__(256) k_halo(void* pop, uint64_t stride, uint32_t count, const HaloArgs h,
+                                              S* buf)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 5ull * h.plane) {
+    return;
+  }
+  const uint32_t k = (uint32_t)(t / h.plane), c = (uint32_t)(t - (uint64_t)k * h.plane);
+  const uint32_t u = c / h.ext_v, v = c - u * h.ext_v;
+  const int64_t cell = h.base + (int64_t)u * h.step_u + (int64_t)v * h.step_v;
+  S* at = ring_at<S>(pop, stride, (int)h.pops[k], ring_slot((uint32_t)cell, h.shift[k], count));
+  if constexpr (PACK) {
+    buf[t] = *at;
+  } else {
+    *at = buf[t];
+  }
+}
+
+// plane of the face (axis, side): the ghost plane for packing, the core
+// boundary plane for unpacking
+olbgfx_status halo_args(const olbgfx_lattice* L, int32_t axis, int32_t side, bool pack, HaloArgs& h)
+{
+  const olbgfx_lattice_desc& D = L->desc;
+  if (axis < 0 || axis > 2 || (side != 0 && side != 1)) {
+    return fail(OLBGFX_ERR_INVALID, "halo face: axis in [0, 2], side 0 (low) or 1 (high)");
+  }
+  if (D.periodic[axis]) {
+    return fail(OLBGFX_ERR_INVALID, "halo faces exist only on non-periodic axes");
+  }
+  const int32_t ext[3] = {D.nx, D.ny, D.nz};
+  const int64_t cs[3] = {(int64_t)L->py * L->pz, (int64_t)L->pz, 1};
+  const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+  // pack reads the ghost plane beyond the face, unpack the core plane inside it
+  const int32_t pos = pack ? (side ? ext[axis] : -1) : (side ? ext[axis] - 1 : 0);
+  int32_t at[3] = {0, 0, 0};
+  at[axis] = pos;
+  h.base = 0;
+  for (int a = 0; a < 3; ++a) {
+    h.base += (int64_t)(at[a] + D.padding) * cs[a];
+  }
+  h.step_u = cs[ua];
+  h.step_v = cs[va];
+  h.ext_v = (uint32_t)ext[va];
+  h.plane = (uint32_t)ext[ua] * (uint32_t)ext[va];
+  // populations crossing the face: c_a = +1 for the high face when packing
+  // and for the low face when unpacking, c_a = -1 otherwise
+  const int want = (pack == (side == 1)) ? 1 : -1;
+  int k = 0;
+  for (int i = 1; i < Q; ++i) {
+    const int c[3] = {CX(i), CY(i), CZ(i)};
+    if (c[axis] == want) {
+      h.pops[k] = (uint32_t)i;
+      h.shift[k] = L->shift[i];
+      ++k;
+    }
+  }
+  return OLBGFX_OK;
+}
+
+template <typename S>
+olbgfx_status halo_launch(olbgfx_lattice* L, const HaloArgs& h, bool pack, void* buf, hipStream_t s)
+{
+  const uint64_t n = 5ull * h.plane;
+  if (n == 0) {
+    return OLBGFX_OK;
+  }
+  const dim3 grid((uint32_t)((n + 255) / 256)), block(256);
+  if (pack) {
+    k_halo<S, true><<<grid, block, 0, s>>>(L->pop, L->stride, L->count, h, (S*)buf);
+  } else {
+    k_halo<S, false><<<grid, block, 0, s>>>(L->pop, L->stride, L->count, h, (S*)buf);
+  }
+  HIP_TRY(hipGetLastError());
+  return OLBGFX_OK;
+}
+
+olbgfx_status halo(olbgfx_lattice* L, int32_t axis, int32_t side, void* buf, void* stream, bool pack)
+{
+  if (check(L) != OLBGFX_OK || !buf) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  HaloArgs h{};
+  if (halo_args(L, axis, side, pack, h) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (L->desc.storage) {
+  case OLBGFX_STORE_F32:
+    return halo_launch<float>(L, h, pack, buf, s);
+  case OLBGFX_STORE_F64:
+    return halo_launch<double>(L, h, pack, buf, s);
+  default:  // F16 and BF16 move as raw 16-bit words
+    return halo_launch<uint16_t>(L, h, pack, buf, s);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_halo_bytes(const olbgfx_lattice* L, int32_t axis, uint64_t* bytes)
+{
+  if (!L || !bytes || axis < 0 || axis > 2) {
+    return fail(OLBGFX_ERR_INVALID, "null argument or axis outside [0, 2]");
+  }
+  const int32_t ext[3] = {L->desc.nx, L->desc.ny, L->desc.nz};
+  const int ua = axis == 0 ? 1 : 0, va = axis == 2 ? 1 : 2;
+  *bytes = 5ull * (uint64_t)ext[ua] * (uint64_t)ext[va] * (uint64_t)L->elem;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_halo_buffer_create(const olbgfx_lattice* L, int32_t axis, void** device_buffer)
+{
+  uint64_t bytes = 0;
+  if (!device_buffer || olbgfx_halo_bytes(L, axis, &bytes) != OLBGFX_OK) {
+    return fail(OLBGFX_ERR_INVALID, "null argument or axis outside [0, 2]");
+  }
+  *device_buffer = nullptr;
+  if (hipMalloc(device_buffer, std::max<uint64_t>(bytes, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    *device_buffer = nullptr;
+    return fail(OLBGFX_ERR_NOMEM, "halo buffer allocation failed");
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_halo_buffer_destroy(void* device_buffer)
+{
+  if (device_buffer) {
+    HIP_TRY(hipFree(device_buffer));
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_halo_pack(olbgfx_lattice* L, int32_t axis, int32_t side, void* device_buffer,
+                               void* stream)
+{
+  return halo(L, axis, side, device_buffer, stream, true);
+}
+
+olbgfx_status olbgfx_halo_unpack(olbgfx_lattice* L, int32_t axis, int32_t side, const void* device_buffer,
+                                 void* stream)
+{
+  return halo(L, axis, side, const_cast<void*>(device_buffer), stream, false);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuner: candidate launch configurations timed on the lattice or ensemble
+// itself; the populations, offsets and statistics are saved before the trials
+// and put back after them, so tuning does not advance the simulation
+
+namespace {
+
+struct SavedState {
+  olbgfx_lattice* L = nullptr;
+  void* pop = nullptr;
+  uint32_t shift[Q] = {};
+  double stats[4] = {};
+  bool stats_valid = false;
+};
+
+void release(std::vector<SavedState>& saved)
+{
+  for (SavedState& s : saved) {
+    if (s.pop) {
+      (void)hipFree(s.pop);
+      s.pop = nullptr;
+    }
+  }
+}
+
+olbgfx_status save_state(olbgfx_lattice* L, hipStream_t stream, SavedState& out)
+{
+  out.L = L;
+  const size_t bytes = (size_t)Q * L->stride * L->elem;
+  if (hipMalloc(&out.pop, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    out.pop = nullptr;
+    return fail(OLBGFX_ERR_NOMEM, "not enough device memory to keep the lattice state while tuning");
+  }
+  HIP_TRY(hipMemcpyAsync(out.pop, L->pop, bytes, hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(out.stats, L->stats_out, sizeof(out.stats), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  std::memcpy(out.shift, L->shift, sizeof(out.shift));
+  out.stats_valid = L->stats_valid;
+  return OLBGFX_OK;
+}
+
+olbgfx_status restore_state(const SavedState& s, hipStream_t stream)
+{
+  olbgfx_lattice* L = s.L;
+  const size_t bytes = (size_t)Q * L->stride * L->elem;
+  HIP_TRY(hipMemcpyAsync(L->pop, s.pop, bytes, hipMemcpyDeviceToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(L->stats_out, s.stats, sizeof(s.stats), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  std::memcpy(L->shift, s.shift, sizeof(L->shift));
+  L->stats_valid = s.stats_valid;
+  return sync_shifts(L);
+}
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n)
+{
+  const unsigned char* p = (const unsigned char*)data;
+  for (size_t i = 0; i < n; ++i) {
+    h = (h ^ p[i]) * 1099511628211ull;
+  }
+  return h;
+}
+
+// extent, padding, periodicity, storage and dynamics mix of a lattice
+std::string tune_key(const olbgfx_lattice* L)
+{
+  uint64_t h = 1469598103934665603ull;
+  h = fnv1a(h, &L->uniform_id, sizeof(L->uniform_id));
+  if (L->uniform_id < 0) {
+    h = fnv1a(h, L->host_mask.data(), L->host_mask.size());
+  }
+  for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
+    h = fnv1a(h, &L->table[k].kind, sizeof(L->table[k].kind));
+  }
+  const olbgfx_lattice_desc& D = L->desc;
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "lattice:%ux%ux%u:p%u:%d%d%d:s%d:%016llx", D.nx, D.ny, D.nz, D.padding,
+                D.periodic[0] ? 1 : 0, D.periodic[1] ? 1 : 0, D.periodic[2] ? 1 : 0, D.storage,
+                (unsigned long long)h);
+  return buf;
+}
+
+std::string tune_key(const olbgfx_ensemble* E)
+{
+  uint64_t h = 1469598103934665603ull;
+  for (const olbgfx_lattice* L : E->members) {
+    const std::string k = tune_key(L);
+    h = fnv1a(h, k.data(), k.size());
+  }
+  char buf[96];
+  std::snprintf(buf, sizeof(buf), "ensemble:%zu:s%d:%016llx", E->members.size(), E->storage,
+                (unsigned long long)h);
+  return buf;
+}
+
+// cache file: one line per key, "key block waves order tiles group_bytes
+// steps_per_group ticket ms"
+bool cache_lookup(const char* path, const std::string& key, olbgfx_tune_trial& t)
+{
+  if (!path || !*path) {
+    return false;
+  }
+  FILE* f = std::fopen(path, "r");
+  if (!f) {
+    return false;
+  }
+  char line[512];
+  bool hit = false;
+  while (std::fgets(line, sizeof(line), f)) {
+    char k[256];
+    unsigned long long gb = 0;
+    olbgfx_tune_trial r{};
+    if (std::sscanf(line, "%255s %u %u %d %u %llu %u %d %lf", k, &r.block_threads, &r.waves_per_simd,
+                    &r.block_order, &r.tiles_per_block, &gb, &r.steps_per_group, &r.ticket,
+                    &r.ms_per_step) == 9 &&
+        key == k) {
+      r.group_bytes = gb;
+      t = r;
+      hit = true;
+    }
+  }
+  std::fclose(f);
+  return hit;
+}
+
+void cache_store(const char* path, const std::string& key, const olbgfx_tune_trial& t)
+{
+  if (!path || !*path) {
+    return;
+  }
+  std::vector<std::string> keep;
+  if (FILE* f = std::fopen(path, "r")) {
+    char line[512];
+    while (std::fgets(line, sizeof(line), f)) {
+      char k[256];
+      if (std::sscanf(line, "%255s", k) == 1 && key != k) {
+        keep.emplace_back(line);
+      }
+    }
+    std::fclose(f);
+  }
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) {
+    return;  // an unwritable cache only costs the next run its trials
+  }
+  for (const std::string& l : keep) {
+    std::fputs(l.c_str(), f);
+  }
+  std::fprintf(f, "%s %u %u %d %u %llu %u %d %.6f\n", key.c_str(), t.block_threads, t.waves_per_simd,
+               t.block_order, t.tiles_per_block, (unsigned long long)t.group_bytes, t.steps_per_group,
+               t.ticket, t.ms_per_step);
+  const bool ok = std::fclose(f) == 0;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+  }
+}
+
+template <typename V, size_t N, size_t M>
+std::vector<V> choices(uint32_t n, const V (&given)[N], const V (&defaults)[M])
+{
+  if (n == 0) {
+    return std::vector<V>(defaults, defaults + M);
+  }
+  return std::vector<V>(given, given + std::min<size_t>(n, N));
+}
+
+struct Trials {
+  std::vector<olbgfx_tune_trial> all;
+  size_t best = 0;
+  void add(const olbgfx_tune_trial& t)
+  {
+    all.push_back(t);
+    if (t.ms_per_step < all[best].ms_per_step) {
+      best = all.size() - 1;
+    }
+  }
+};
+
+std::vector<olbgfx_tune_trial> fastest(std::vector<olbgfx_tune_trial> all, size_t n)
+{
+  std::stable_sort(all.begin(), all.end(), [](const olbgfx_tune_trial& x, const olbgfx_tune_trial& y) {
+    return x.ms_per_step < y.ms_per_step;
+  });
+  all.resize(std::min(n, all.size()));
+  return all;
+}
+
+void report(const Trials& tr, bool cached, const olbgfx_tune_trial& best, olbgfx_tune_result* out)
+{
+  if (!out) {
+    return;
+  }
+  std::memset(out, 0, sizeof(*out));
+  out->best = best;
+  out->cached = cached ? 1 : 0;
+  out->ntrials = (uint32_t)std::min<size_t>(tr.all.size(), OLBGFX_TUNE_TRIALS);
+  for (uint32_t k = 0; k < out->ntrials; ++k) {
+    out->trials[k] = tr.all[k];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_autotune_lattice(olbgfx_lattice* L, uint32_t steps, const olbgfx_tune_space* space,
+                                      const char* cache_path, void* stream, olbgfx_tune_result* out)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (steps == 0) {
+    return fail(OLBGFX_ERR_INVALID, "the autotuner needs at least one timed step per trial");
+  }
+  static const uint32_t kBlocks[] = {64, 128, 256};
+  static const uint32_t kWaves[] = {0, 2, 3, 4, 5, 6};
+  static const int32_t kOrders[] = {-1, 0, 2, 3, 4, 5, 6, 8};
+  const olbgfx_tune_space none{};
+  const olbgfx_tune_space& sp = space ? *space : none;
+  const std::vector<uint32_t> blocks = choices(sp.n_block_threads, sp.block_threads, kBlocks);
+  const std::vector<uint32_t> waves = choices(sp.n_waves, sp.waves_per_simd, kWaves);
+  const std::vector<int32_t> orders = choices(sp.n_orders, sp.block_order, kOrders);
+  const std::string key = tune_key(L);
+  Trials tr;
+  olbgfx_tune_trial best{};
+  if (cache_lookup(cache_path, key, best)) {
+    olbgfx_status st = olbgfx_set_launch(L, best.block_threads, best.waves_per_simd);
+    if (st == OLBGFX_OK) {
+      st = olbgfx_set_block_order(L, best.block_order);
+    }
+    if (st == OLBGFX_OK) {
+      report(tr, true, best, out);
+    }
+    return st;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<SavedState> saved(1);
+  olbgfx_status st = save_state(L, s, saved[0]);
+  auto trial = [&](uint32_t b, uint32_t w, int32_t o) -> olbgfx_status {
+    olbgfx_status r = olbgfx_set_launch(L, b, w);
+    if (r == OLBGFX_OK) r = olbgfx_set_block_order(L, o);
+    if (r == OLBGFX_OK) r = olbgfx_collide_and_stream(L, 1, stream);
+    float ms = 0;
+    if (r == OLBGFX_OK) r = olbgfx_time_steps(L, steps, stream, &ms);
+    if (r == OLBGFX_OK) {
+      olbgfx_tune_trial t{};
+      t.block_threads = b;
+      t.waves_per_simd = w;
+      t.block_order = o;
+      t.tiles_per_block = 1;
+      t.steps_per_group = 1;
+      t.ms_per_step = (double)ms / steps;
+      tr.add(t);
+    }
+    return r;
+  };
+  for (size_t i = 0; i < blocks.size() && st == OLBGFX_OK; ++i) {
+    for (size_t j = 0; j < waves.size() && st == OLBGFX_OK; ++j) {
+      st = trial(blocks[i], waves[j], -1);
+    }
+  }
+  // block orders for the three fastest shapes
+  const std::vector<olbgfx_tune_trial> shapes = fastest(tr.all, 3);
+  for (size_t m = 0; m < shapes.size() && st == OLBGFX_OK; ++m) {
+    for (size_t k = 0; k < orders.size() && st == OLBGFX_OK; ++k) {
+      if (orders[k] != -1) {
+        st = trial(shapes[m].block_threads, shapes[m].waves_per_simd, orders[k]);
+      }
+    }
+  }
+  if (saved[0].pop) {
+    const std::string err = st == OLBGFX_OK ? std::string() : g_error;
+    const olbgfx_status rs = restore_state(saved[0], s);
+    if (st == OLBGFX_OK) {
+      st = rs;
+    } else {
+      g_error = err;
+    }
+  }
+  release(saved);
+  if (st != OLBGFX_OK) {
+    return st;
+  }
+  if (tr.all.empty()) {
+    return fail(OLBGFX_ERR_INVALID, "no candidate configurations");
+  }
+  best = tr.all[tr.best];
+  st = olbgfx_set_launch(L, best.block_threads, best.waves_per_simd);
+  if (st == OLBGFX_OK) st = olbgfx_set_block_order(L, best.block_order);
+  if (st == OLBGFX_OK) {
+    cache_store(cache_path, key, best);
+    report(tr, false, best, out);
+  }
+  return st;
+}
+
+olbgfx_status olbgfx_autotune_ensemble(olbgfx_ensemble* E, uint32_t steps, const olbgfx_tune_space* space,
+                                       const char* cache_path, void* stream, olbgfx_tune_result* out)
+{
+  if (!E || E->members.empty()) {
+    return fail(OLBGFX_ERR_INVALID, "null or empty ensemble");
+  }
+  static const uint32_t kBlocks[] = {128, 256};
+  static const uint32_t kWaves[] = {0, 3, 4, 5};
+  static const uint32_t kTiles[] = {1, 2, 4};
+  static const uint64_t kGroups[] = {0, 96ull << 20, 160ull << 20};
+  static const uint32_t kPerGroup[] = {1, 4, 8};
+  static const int32_t kTickets[] = {0, 1};
+  const olbgfx_tune_space none{};
+  const olbgfx_tune_space& sp = space ? *space : none;
+  const std::vector<uint32_t> blocks = choices(sp.n_block_threads, sp.block_threads, kBlocks);
+  const std::vector<uint32_t> waves = choices(sp.n_waves, sp.waves_per_simd, kWaves);
+  const std::vector<uint32_t> tiles = choices(sp.n_tiles, sp.tiles_per_block, kTiles);
+  const std::vector<uint64_t> groups = choices(sp.n_group_bytes, sp.group_bytes, kGroups);
+  const std::vector<uint32_t> per_group = choices(sp.n_steps_per_group, sp.steps_per_group, kPerGroup);
+  const std::vector<int32_t> tickets = choices(sp.n_tickets, sp.ticket, kTickets);
+  static const int32_t kOrders[] = {-1, 0, 2, 3, 4, 5, 6, 8};
+  const std::vector<int32_t> orders = choices(sp.n_orders, sp.block_order, kOrders);
+  const std::string key = tune_key(E);
+  Trials tr;
+  olbgfx_tune_trial best{};
+  auto apply = [&](const olbgfx_tune_trial& t) -> olbgfx_status {
+    olbgfx_status r = olbgfx_ensemble_set_launch(E, t.block_threads, t.waves_per_simd, t.tiles_per_block);
+    if (r == OLBGFX_OK) r = olbgfx_ensemble_set_block_order(E, t.block_order);
+    if (r == OLBGFX_OK) r = olbgfx_ensemble_set_schedule(E, t.group_bytes, t.steps_per_group, t.ticket);
+    return r;
+  };
+  if (cache_lookup(cache_path, key, best)) {
+    const olbgfx_status st = apply(best);
+    if (st == OLBGFX_OK) {
+      report(tr, true, best, out);
+    }
+    return st;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t span = std::max(8u, steps);
+  std::vector<SavedState> saved(E->members.size());
+  olbgfx_status st = OLBGFX_OK;
+  for (size_t k = 0; k < E->members.size() && st == OLBGFX_OK; ++k) {
+    st = save_state(E->members[k], s, saved[k]);
+  }
+  auto trial = [&](const olbgfx_tune_trial& cand) -> olbgfx_status {
+    olbgfx_status r = apply(cand);
+    if (r == OLBGFX_OK) r = olbgfx_ensemble_step(E, cand.steps_per_group, stream);
+    float ms = 0;
+    if (r == OLBGFX_OK) r = olbgfx_ensemble_time(E, 0, span, stream, &ms);
+    if (r == OLBGFX_OK) {
+      olbgfx_tune_trial t = cand;
+      t.ms_per_step = (double)ms / span;
+      tr.add(t);
+    }
+    return r;
+  };
+  for (size_t i = 0; i < blocks.size() && st == OLBGFX_OK; ++i) {
+    for (size_t j = 0; j < waves.size() && st == OLBGFX_OK; ++j) {
+      for (size_t k = 0; k < tiles.size() && st == OLBGFX_OK; ++k) {
+        olbgfx_tune_trial c{};
+        c.block_threads = blocks[i];
+        c.waves_per_simd = waves[j];
+        c.block_order = -1;
+        c.tiles_per_block = tiles[k];
+        c.group_bytes = 0;
+        c.steps_per_group = 1;
+        c.ticket = 0;
+        st = trial(c);
+      }
+    }
+  }
+  const std::vector<olbgfx_tune_trial> shapes = fastest(tr.all, 3);
+  for (size_t m = 0; m < shapes.size() && st == OLBGFX_OK; ++m) {
+    for (size_t k = 0; k < orders.size() && st == OLBGFX_OK; ++k) {
+      if (orders[k] != -1) {
+        olbgfx_tune_trial c = shapes[m];
+        c.block_order = orders[k];
+        st = trial(c);
+      }
+    }
+  }
+  if (st == OLBGFX_OK && !tr.all.empty()) {
+    const olbgfx_tune_trial first = tr.all[tr.best];
+    for (size_t i = 0; i < groups.size() && st == OLBGFX_OK; ++i) {
+      for (size_t j = 0; j < per_group.size() && st == OLBGFX_OK; ++j) {
+        for (size_t k = 0; k < tickets.size() && st == OLBGFX_OK; ++k) {
+          if (groups[i] == 0 && per_group[j] == 1 && tickets[k] == 0) {
+            continue;  // the launch phase already timed this schedule
+          }
+          olbgfx_tune_trial c = first;
+          c.group_bytes = groups[i];
+          c.steps_per_group = per_group[j];
+          c.ticket = tickets[k] ? 1 : 0;
+          st = trial(c);
+        }
+      }
+    }
+  }
+  const std::string err = st == OLBGFX_OK ? std::string() : g_error;
+  for (size_t k = 0; k < saved.size(); ++k) {
+    if (saved[k].pop) {
+      const olbgfx_status rs = restore_state(saved[k], s);
+      if (st == OLBGFX_OK) {
+        st = rs;
+      }
+    }
+  }
+  release(saved);
+  if (st != OLBGFX_OK) {
+    if (!err.empty()) {
+      g_error = err;
+    }
+    return st;
+  }
+  if (tr.all.empty()) {
+    return fail(OLBGFX_ERR_INVALID, "no candidate configurations");
+  }
+  best = tr.all[tr.best];
+  st = apply(best);
+  if (st == OLBGFX_OK) {
+    cache_store(cache_path, key, best);
+    report(tr, false, best, out);
+  }
+  return st;
 }
 
 }  // extern "C"

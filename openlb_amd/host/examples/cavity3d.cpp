@@ -7,7 +7,7 @@
  * benchmark does (cavity3d.cpp:225-234).
  *
  *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
- *            [--STORAGE f32|f16|bf16] [--DUMP file]
+ *            [--STORAGE f32|f16|bf16] [--DUMP file] [--TUNE_CACHE file]
  *
  * --DUMP writes the materials and the populations of the padded block after
  * the run (logical order, float32) for the tests.
@@ -32,6 +32,7 @@ struct Parameters {
   T reynolds = T(100);
   Storage storage = Storage::F32;
   std::string dump;
+  std::string tuneCache;  // "" = no autotuning
 };
 
 static Parameters fromCLI(int argc, char** argv)
@@ -44,6 +45,7 @@ static Parameters fromCLI(int argc, char** argv)
     else if (k == "--LID_VELOCITY") p.lidVelocity = std::stof(v);
     else if (k == "--REYNOLDS") p.reynolds = std::stof(v);
     else if (k == "--DUMP") p.dump = v;
+    else if (k == "--TUNE_CACHE") p.tuneCache = v;
     else if (k == "--STORAGE") {
       p.storage = v == "f16" ? Storage::F16 : (v == "bf16" ? Storage::BF16 : Storage::F32);
     } else {
@@ -92,6 +94,11 @@ int main(int argc, char** argv)
     SuperLattice<T, DESCRIPTOR> lattice(geometry, p.storage);
     prepareLattice(lattice, geometry, p);
     setInitialValues(lattice, geometry, p);
+    if (!p.tuneCache.empty()) {
+      const olbgfx_tune_trial t = lattice.autotune(p.tuneCache);
+      std::cerr << "autotune: block " << t.block_threads << ", waves/SIMD " << t.waves_per_simd
+                << ", order " << t.block_order << ", " << t.ms_per_step << " ms/step" << std::endl;
+    }
 
     for (int iT = 0; iT < 10; ++iT) {
       lattice.collideAndStream();

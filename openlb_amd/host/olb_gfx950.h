@@ -11,6 +11,7 @@
  *   SuperLattice::initialize, collideAndStream core/superLattice.hh:248-266, 663-725
  *   setStatisticsOn/Off, getStatistics         core/superLattice.h, core/latticeStatistics.h:43-127
  *   setProcessingContext(Evaluation)           core/superLattice.hh:184-195
+ *   autotune (an addition of this platform)    no counterpart; OpenLB fixes launch shapes
  *
  * Everything below calls the C ABI of include/olbgfx.h; no HIP header is
  * included, so a host translation unit builds with any C++17 compiler and
@@ -366,6 +367,21 @@ public:
     }
     check(olbgfx_collide_and_stream(handle(), 1, nullptr));
     _hostValid = false;
+  }
+
+  /// Launch autotuner of the platform (no reference counterpart): times the
+  /// candidate launch shapes on this lattice, applies the fastest and puts
+  /// the populations back, so the simulation does not advance.  An empty
+  /// cachePath tunes without a cache file.
+  olbgfx_tune_trial autotune(const std::string& cachePath = "", std::uint32_t steps = 4)
+  {
+    if (!_initialized || _dirty) {
+      initialize();
+    }
+    olbgfx_tune_result r;
+    check(olbgfx_autotune_lattice(handle(), steps, nullptr, cachePath.empty() ? nullptr : cachePath.c_str(),
+                                  nullptr, &r));
+    return r.best;
   }
 
   void setStatisticsOn() { setStatistics(true); }

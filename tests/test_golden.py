@@ -92,3 +92,21 @@ def test_oracle_cavity_res64_1000_steps_matches_reference():
     # of u_x, u_y to 1e-2 (4e-8 per cell)
     np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=1e-4)
     np.testing.assert_allclose(sums[1:3], d["fluid_sums"][1:3], atol=1e-2)
+
+
+def test_porous_geometry_matches_reference_full_size():
+    """the 384x128x128 sphere pack (2087 spheres, r = 6) marks exactly the
+    cells OpenLB's FP32 run marks (IndicatorSphere3D over every periodic
+    image): CRC-32 of the core materials and per-material counts"""
+    import zlib
+
+    from openlb_amd import cases
+
+    meta, d = load("porous384x128x128_5000steps_f32")
+    c = cases.porous_channel((meta["nx"], meta["ny"], meta["nz"]), radius=6.0, porosity=0.7,
+                             seed=20261202, force=1e-6, tau=0.8, magic=3.0 / 16.0, dtype=np.float32)
+    assert len(c.info["spheres"]) == meta["nspheres"]
+    p = c.padding
+    core = np.ascontiguousarray(c.materials[p:-p, p:-p, p:-p])
+    np.testing.assert_array_equal(np.bincount(core.reshape(-1), minlength=4), d["material_counts"])
+    assert zlib.crc32(core.tobytes()) == int(d["core_materials_crc32"])

@@ -139,17 +139,46 @@ def test_mixed_storage_is_rejected():
         Ensemble([a, b])
 
 
-def test_autotuners_keep_results(tmp_path, monkeypatch):
-    monkeypatch.setenv("OLBGFX_TUNE_CACHE", str(tmp_path / "tune.json"))
+def test_autotuners_keep_state_and_cache(tmp_path, monkeypatch):
+    """tuning puts populations, offsets and statistics back, and a second
+    call with the same key is served from the cache"""
+    monkeypatch.setenv("OLBGFX_TUNE_CACHE", str(tmp_path / "tune.txt"))
     _, a = _members("f32", sizes=[10, 12])
     _, b = _members("f32", sizes=[10, 12])
+    for lat in (a[0], b[0]):
+        lat.set_statistics(True)
+        lat.collide_and_stream(3)
     res = autotune.tune_lattice(a[0], steps=2, blocks=(64, 256), waves=(0, 3), orders=(-1, 2))
-    assert res["block"] in (64, 256) and res["order"] in (-1, 2) and len(res["trials"]) == 5
-    b[0].collide_and_stream(5 * (1 + 2))  # five trials of one untimed + two timed steps
+    assert res["block"] in (64, 256) and res["order"] in (-1, 2)
+    assert len(res["trials"]) == 4 + 3 and not res["cached"]  # orders for the three fastest pairs
     np.testing.assert_array_equal(a[0].get_populations(), b[0].get_populations())
-    ens = Ensemble(a)
-    r2 = autotune.tune_ensemble(ens, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2),
+    np.testing.assert_array_equal(a[0].ring_shifts(), b[0].ring_shifts())
+    assert a[0].get_statistics() == b[0].get_statistics()
+    ens_a, ens_b = Ensemble(a), Ensemble(b)
+    r2 = autotune.tune_ensemble(ens_a, steps=1, blocks=(128,), waves=(0, 4), tiles=(1, 2), orders=(-1, 3),
                                 group_bytes=(0, 1), steps_per_group=(1, 2), tickets=(False,))
-    assert len(r2["trials"]) == 4 + 4
+    # 4 shapes, order 3 for the three fastest, 3 schedules (one group, one
+    # step per group is timed once)
+    assert len(r2["trials"]) == 4 + 3 + 3
+    ens_a.step(5)
+    ens_b.step(5)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+        np.testing.assert_array_equal(x.ring_shifts(), y.ring_shifts())
     cached = autotune.tune_lattice(a[0], steps=2)
-    assert cached.get("cached")
+    assert cached["cached"] and cached["block"] == res["block"] and cached["order"] == res["order"]
+    assert (tmp_path / "tune.txt").read_text().count("\n") == 2
+
+
+@pytest.mark.parametrize("order,tiles", [(0, 1), (3, 2), (6, 1)])
+def test_ensemble_block_orders_give_identical_results(order, tiles):
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ea, eb = Ensemble(a), Ensemble(b)
+    ea.set_launch(128, 0, tiles)
+    eb.set_launch(128, 0, tiles)
+    eb.set_block_order(order)
+    ea.step(9)
+    eb.step(9)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())

@@ -1,6 +1,8 @@
 """The full-size workloads on the GPU against reductions of the reference's
 own FP32 CPU run (tests/golden/make_fixtures.py, FULL): mid-plane velocity,
 the populations of 256 seeded fluid cells and FP64 sums over the fluid."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -44,6 +46,9 @@ def test_full_size_fp32_matches_reference(name):
 
     meta, d = load(name)
     case = _case(meta)
+    p = case.padding
+    core = np.ascontiguousarray(case.materials[p:-p, p:-p, p:-p])
+    assert zlib.crc32(core.tobytes()) == int(d["core_materials_crc32"])
     lat = Lattice(case.extent, case.padding, case.periodic, "f32")
     case.apply(lat)
     for mat, dyn in dynamics_for(meta, "f32").items():
