@@ -48,6 +48,7 @@ def parse_args():
     ap.add_argument("--no-porous", action="store_true")
     ap.add_argument("--porous-steps", type=int, default=500)
     ap.add_argument("--no-slab", action="store_true")
+    ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     return ap.parse_args()
@@ -109,7 +110,7 @@ def time_storage(torch, size, storage, steps, warmup, block, tune=True):
 
 
 def time_ensemble(torch, storage, steps, warmup, tune=True):
-    """config 5: 64 lid-driven cavities, N ~ U{32..128}, U ~ U[0.02, 0.1],
+    """the ensemble of 64 lid-driven cavities, N ~ U{32..128}, U ~ U[0.02, 0.1],
     Re = 100; persistent kernel per step (eager and graph-replayed) against
     launching every cavity separately"""
     from openlb_amd import Ensemble, Lattice, autotune, cases
@@ -158,7 +159,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
 
 
 def accuracy(torch, size, steps):
-    """config 3: TGV at size^3 run `steps` steps in FP64, FP32, FP16 and BF16
+    """TGV at size^3 run `steps` steps in FP64, FP32, FP16 and BF16
     storage from the same analytic start; velocity error of each against the
     FP64 run (relative L2 over the core, max abs relative to u0), and the FP64
     kinetic-energy decay against the analytic rate"""
@@ -226,6 +227,27 @@ def time_porous(torch, steps, storages=("f32", "f16")):
         lat.close()
     res["porosity"] = case.info["porosity"]
     res["spheres"] = int(len(case.info["spheres"]))
+    return res
+
+
+def time_smagorinsky(torch, size, steps, storages=("f32", "f16")):
+    """the bulk dynamics of OpenLB's tgv3d example: SmagorinskyBGKdynamics
+    (C = 0.1) on the Taylor-Green vortex at size^3, tau = 0.505"""
+    from openlb_amd import SMAGORINSKY_BGK, Dynamics, Lattice, autotune, cases
+
+    case = cases.tgv(size, u0=0.04, tau=0.505, dtype=np.float32)
+    case.dynamics[1] = Dynamics(SMAGORINSKY_BGK, omega=1 / 0.505, smagorinsky=0.1)
+    handle = torch.cuda.current_stream().cuda_stream
+    res = {}
+    for st in storages:
+        lat = Lattice(case.extent, case.padding, case.periodic, st)
+        case.apply(lat)
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+        lat.collide_and_stream(5, handle)
+        ms = lat.time_steps(steps, handle) / steps
+        res[st] = {"glups": size ** 3 / (ms * 1e-3) / 1e9, "ms_per_step": ms,
+                   "launch": {k: tuned[k] for k in ("block", "waves", "order")}}
+        lat.close()
     return res
 
 
@@ -375,6 +397,10 @@ def main():
     if porous:
         print(f"# porous: f32 {porous['f32']['glups']:.2f} GLUPS, f16 {porous['f16']['glups']:.2f} GLUPS",
               file=sys.stderr)
+    les = None if args.no_les else time_smagorinsky(torch, args.size, args.steps)
+    if les:
+        print("# smagorinsky: " + ", ".join(f"{k} {v['glups']:.2f} GLUPS" for k, v in les.items()),
+              file=sys.stderr)
     slab = None if args.no_slab else time_slab(torch, args.steps)
     if slab:
         print(f"# slab {slab['shape']}: {slab['glups']:.2f} GLUPS with halo planes, "
@@ -428,6 +454,7 @@ def main():
                      for k, v in ens.items()},
         "porous": porous,
         "slab": slab,
+        "smagorinsky": les,
         "accuracy": acc,
         "roofline": roofline,
         "cpu_baseline": base,

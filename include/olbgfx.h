@@ -49,6 +49,24 @@ typedef int32_t olbgfx_status;
 #define OLBGFX_FORCED_TRT           4  /* dynamics.h:543-549 ForcedTRTdynamics   */
 #define OLBGFX_BOUNCE_BACK          5  /* dynamics.h:558-568 BounceBack          */
 #define OLBGFX_BOUNCE_BACK_VELOCITY 6  /* dynamics.h:635-640 BounceBackVelocity  */
+/* LocalVelocity boundaries (boundary/localVelocity3D.h:36-120); both need the
+ * cell's discrete normal (olbgfx_set_cell_normals) and take the wall
+ * velocity from wall_velocity or the VELOCITY field:
+ *   LOCAL_VELOCITY: a flat face, CombinedRLBdynamics<T,D,BGKdynamics,
+ *     RegularizedVelocityBoundaryTuple<direction,orientation>>
+ *     (dynamics.h:476-526) with the direction and orientation of its normal;
+ *   FIXED_VELOCITY: BGKdynamics with FixedVelocityBoundaryTuple
+ *     (momenta/aliases.h:215-220); on a cell whose normal has two non-zero
+ *     entries (external edge) or three (external corner) the step also runs
+ *     OuterVelocityEdgeProcessor3D / OuterVelocityCornerProcessor3D after
+ *     streaming (boundary/boundaryPostProcessors3D.hh:164-294), edges before
+ *     corners as their priorities 0 and 1 order them. */
+#define OLBGFX_LOCAL_VELOCITY       7
+#define OLBGFX_FIXED_VELOCITY       8
+/* SmagorinskyBGKdynamics (dynamics/smagorinskyBGKdynamics.h:37-44): BGK with
+ * the effective relaxation of collision::SmagorinskyEffectiveOmega
+ * (dynamics/collisionLES.h:59-89), constant `smagorinsky` */
+#define OLBGFX_SMAGORINSKY_BGK      9
 
 /* per-cell fields */
 #define OLBGFX_FIELD_VELOCITY 0  /* descriptors::VELOCITY, 3 values per cell */
@@ -71,6 +89,7 @@ typedef struct {
   double force[3];           /* uniform FORCE unless a FORCE field is set     */
   double wall_velocity[3];   /* uniform VELOCITY unless a VELOCITY field is set */
   double wall_rho;           /* uniform RHO unless a RHO field is set (1)     */
+  double smagorinsky;        /* collision::LES::SMAGORINSKY                    */
 } olbgfx_dynamics_desc;
 
 /* ---- lattice lifetime --------------------------------------------------
@@ -112,6 +131,13 @@ olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* lattice, const uint8_t* i
  * field so the uniform dynamics value applies again. */
 olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* lattice, int32_t field,
                                     int32_t value_type, const void* values);
+
+/* discrete normal of every cell (3 x int8 per padded cell, CellID order,
+ * entries in {-1, 0, 1}, pointing out of the fluid), as boundary::set
+ * computes it (geometry/discreteNormals.hh:176, boundary/setBoundary.h:76-100);
+ * NULL removes them.  Read by LOCAL_VELOCITY and FIXED_VELOCITY cells; a step
+ * fails with OLBGFX_ERR_INVALID if a LOCAL_VELOCITY cell has no flat normal. */
+olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* lattice, const int8_t* normals);
 
 /* ---- populations -----------------------------------------------------------
  * iniEquilibrium(indicator, rho, u) (core/blockLattice.hh:202-236,

@@ -30,6 +30,7 @@ LIB_PATH = os.path.join(_HERE, "libolbgfx.so")
 STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
 VALUE_F32, VALUE_F64 = 0, 1
 NO_DYNAMICS, BGK, TRT, FORCED_BGK, FORCED_TRT, BOUNCE_BACK, BOUNCE_BACK_VELOCITY = range(7)
+LOCAL_VELOCITY, FIXED_VELOCITY, SMAGORINSKY_BGK = 7, 8, 9
 FIELD_VELOCITY, FIELD_RHO, FIELD_FORCE = 0, 1, 2
 MAX_DYNAMICS = 16
 Q = 19
@@ -72,6 +73,7 @@ class _DynamicsDesc(ctypes.Structure):
         ("force", ctypes.c_double * 3),
         ("wall_velocity", ctypes.c_double * 3),
         ("wall_rho", ctypes.c_double),
+        ("smagorinsky", ctypes.c_double),
     ]
 
 
@@ -99,6 +101,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_set_dynamics": [vp, u32, ctypes.POINTER(_DynamicsDesc)],
         "olbgfx_set_dynamics_mask": [vp, vp],
         "olbgfx_set_cell_field": [vp, i32, i32, vp],
+        "olbgfx_set_cell_normals": [vp, vp],
         "olbgfx_ini_equilibrium": [vp, vp, i32, vp, vp],
         "olbgfx_get_populations": [vp, i32, vp],
         "olbgfx_set_populations": [vp, i32, vp],
@@ -160,6 +163,7 @@ class Dynamics:
     force: tuple = (0.0, 0.0, 0.0)
     wall_velocity: tuple = (0.0, 0.0, 0.0)
     wall_rho: float = 1.0
+    smagorinsky: float = 0.0
 
     def _desc(self) -> _DynamicsDesc:
         d = _DynamicsDesc()
@@ -170,6 +174,7 @@ class Dynamics:
             d.force[a] = float(self.force[a])
             d.wall_velocity[a] = float(self.wall_velocity[a])
         d.wall_rho = float(self.wall_rho)
+        d.smagorinsky = float(self.smagorinsky)
         return d
 
 
@@ -259,6 +264,17 @@ class Lattice:
         _check(library().olbgfx_set_cell_field(self._h, field, self._value, _ptr(v)))
 
     # -- populations -------------------------------------------------------------
+    def set_cell_normals(self, normals: np.ndarray | None) -> None:
+        """discrete normal per cell (3 x int8 in {-1, 0, 1}, CellID order), as
+        boundary::set computes it; None removes them"""
+        if normals is None:
+            _check(library().olbgfx_set_cell_normals(self._h, None))
+            return
+        v = np.ascontiguousarray(normals, dtype=np.int8).reshape(-1)
+        if v.size != 3 * self.ncells:
+            raise ValueError("normals must hold 3 entries per cell")
+        _check(library().olbgfx_set_cell_normals(self._h, _ptr(v)))
+
     def ini_equilibrium(self, rho: np.ndarray, u: np.ndarray, sel: np.ndarray | None = None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)
         u = np.ascontiguousarray(u, dtype=self._dtype).reshape(-1)

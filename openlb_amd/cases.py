@@ -17,7 +17,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FORCED_TRT, NO_DYNAMICS, Dynamics
+from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT, LOCAL_VELOCITY,
+               NO_DYNAMICS, Dynamics)
 
 __all__ = [
     "Case",
@@ -62,26 +63,65 @@ class Case:
         return int(np.count_nonzero(self.materials))
 
     def dynamics_slots(self):
-        """material -> dynamics table slot (slot 0 stays NoDynamics)."""
+        """material -> dynamics table slot (slot 0 stays NoDynamics).  A
+        LocalVelocity material takes two slots, keyed (material, "flat") for
+        its faces and (material, "fixed") for its edges and corners."""
         slots = {}
         table = [Dynamics(NO_DYNAMICS)]
         for mat in sorted(self.dynamics):
-            table.append(self.dynamics[mat])
-            slots[mat] = len(table) - 1
+            d = self.dynamics[mat]
+            if isinstance(d, LocalVelocity):
+                table.append(Dynamics(LOCAL_VELOCITY, omega=d.omega, wall_velocity=d.wall_velocity))
+                slots[(mat, "flat")] = len(table) - 1
+                table.append(Dynamics(FIXED_VELOCITY, omega=d.omega, wall_velocity=d.wall_velocity))
+                slots[(mat, "fixed")] = len(table) - 1
+            else:
+                table.append(d)
+                slots[mat] = len(table) - 1
         return slots, table
+
+    def local_velocity_materials(self) -> tuple:
+        return tuple(m for m, d in sorted(self.dynamics.items()) if isinstance(d, LocalVelocity))
+
+    def cell_normals(self):
+        """(type, normal) per cell for the LocalVelocity materials, or None"""
+        lv = self.local_velocity_materials()
+        if not lv:
+            return None
+        if "_normals" not in self.info:
+            self.info["_normals"] = discrete_normals(self.materials, lv)
+        return self.info["_normals"]
 
     def dynamics_mask(self) -> np.ndarray:
         slots, _ = self.dynamics_slots()
         lut = np.zeros(256, dtype=np.uint8)
         for mat, slot in slots.items():
-            lut[mat] = slot
-        return lut[self.materials.reshape(-1)]
+            if not isinstance(mat, tuple):
+                lut[mat] = slot
+        mask = lut[self.materials.reshape(-1)]
+        normals = self.cell_normals()
+        if normals is not None:
+            types = normals[0].reshape(-1)
+            mats = self.materials.reshape(-1)
+            for m in self.local_velocity_materials():
+                on = mats == m
+                flat = on & (types == FLAT)
+                fixed = on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))
+                if np.any(on & ~flat & ~fixed):
+                    raise ValueError("LocalVelocity supports flat faces, external edges and external "
+                                     "corners; this geometry has other boundary cells")
+                mask[flat] = slots[(m, "flat")]
+                mask[fixed] = slots[(m, "fixed")]
+        return mask
 
     def apply(self, lattice) -> None:
         slots, table = self.dynamics_slots()
         for slot, dyn in enumerate(table):
             lattice.set_dynamics(slot, dyn)
         lattice.set_dynamics_mask(self.dynamics_mask())
+        normals = self.cell_normals()
+        if normals is not None:
+            lattice.set_cell_normals(normals[1])
         if self.velocity_field is not None:
             lattice.set_cell_field(FIELD_VELOCITY, self.velocity_field)
         if self.rho is not None:
@@ -93,6 +133,87 @@ class Case:
         m = np.zeros(self.shape, dtype=bool)
         m[p:-p, p:-p, p:-p] = True
         return m
+
+
+@dataclass
+class LocalVelocity:
+    """boundary::set<boundary::LocalVelocity<T, D, BGKdynamics>> on a material
+    (boundary/localVelocity3D.h:36-120): flat faces collide as
+    CombinedRLBdynamics with RegularizedVelocityBoundaryTuple, external edges
+    and corners as BGK with FixedVelocityBoundaryTuple followed by the outer
+    edge and corner post-processors.  omega of the BGK part; the wall
+    velocity is the VELOCITY the case defines on the material."""
+
+    omega: float
+    wall_velocity: tuple = (0.0, 0.0, 0.0)
+
+
+# DiscreteNormalType (geometry/discreteNormals.h:33-39)
+FLAT, EXTERNAL_CORNER, INTERNAL_CORNER, EXTERNAL_EDGE, INTERNAL_EDGE = 0, 1, 2, 3, 4
+
+
+def discrete_normals(materials: np.ndarray, boundary, fluid: int = 1, outside: int = 0):
+    """Type and outward discrete normal of every cell whose material is in
+    `boundary`, by the neighbourhood rules of computeBoundaryTypeAndNormal
+    (geometry/discreteNormals.hh:176-1190) for the configurations found on
+    box walls; B means neither fluid nor outside:
+      flat (FLAT): the four in-plane axis neighbours are B and one neighbour
+        along the remaining axis is fluid; the normal points away from it;
+      external edge (EXTERNAL_EDGE), along axis p: both neighbours along p are
+        B, the transverse neighbours s_a e_a and s_b e_b are B, the cells
+        +-e_p + s_b e_b are not fluid and the diagonal s_a e_a + s_b e_b is
+        fluid; normal -(s_a e_a + s_b e_b);
+      external corner (EXTERNAL_CORNER): for signs s, the six cells s_x e_x,
+        s_y e_y, s_z e_z and their pairwise sums are B and s is fluid; normal -s.
+    Cells matching none or several of these (inner edges and corners) get
+    type -2.  Returns (type per cell, -1 off the boundary; normals, int8 x 3)."""
+    m = np.asarray(materials)
+    shape = m.shape
+    P = np.pad(m, 2, constant_values=outside)
+    isf = P == fluid
+    isb = (P != fluid) & (P != outside)
+    types = np.full(shape, -1, dtype=np.int8)
+    normals = np.zeros(shape + (3,), dtype=np.int8)
+    cells = np.argwhere(np.isin(m, list(boundary)))
+
+    def at(arr, c, d):
+        return arr[c[0] + 2 + d[0], c[1] + 2 + d[1], c[2] + 2 + d[2]]
+
+    E = np.eye(3, dtype=int)
+    for c in cells:
+        found = []
+        for a in range(3):
+            o1, o2 = E[(a + 1) % 3], E[(a + 2) % 3]
+            if all(at(isb, c, v) for v in (o1, -o1, o2, -o2)):
+                for s in (1, -1):
+                    if at(isf, c, s * E[a]):
+                        found.append((FLAT, -s * E[a]))
+        for p in range(3):
+            a_ax, b_ax = (p + 1) % 3, (p + 2) % 3
+            if not (at(isb, c, E[p]) and at(isb, c, -E[p])):
+                continue
+            for sa in (1, -1):
+                for sb in (1, -1):
+                    for ea, eb in ((sa * E[a_ax], sb * E[b_ax]), (sb * E[b_ax], sa * E[a_ax])):
+                        if (at(isb, c, eb) and not at(isf, c, E[p] + eb) and not at(isf, c, -E[p] + eb)
+                                and at(isb, c, ea) and at(isf, c, ea + eb)):
+                            found.append((EXTERNAL_EDGE, -(ea + eb)))
+        for sx in (1, -1):
+            for sy in (1, -1):
+                for sz in (1, -1):
+                    s3 = np.array([sx, sy, sz])
+                    ex, ey, ez = sx * E[0], sy * E[1], sz * E[2]
+                    if (all(at(isb, c, v) for v in (ex, ey, ez, ex + ey, ex + ez, ey + ez))
+                            and at(isf, c, s3)):
+                        found.append((EXTERNAL_CORNER, -s3))
+        uniq = {(t, tuple(int(v) for v in n)) for t, n in found}
+        if len(uniq) == 1:
+            t, n = uniq.pop()
+            types[tuple(c)] = t
+            normals[tuple(c)] = n
+        else:
+            types[tuple(c)] = -2
+    return types, normals
 
 
 def _padded(extent, padding):
@@ -307,6 +428,20 @@ def ensemble_cavities(count: int = 64, nmin: int = 32, nmax: int = 128, umin: fl
         u = umin + (umax - umin) * rng.uniform()
         out.append(box_cavity(n, u_lid=u, re=re, padding=padding, dtype=dtype))
     return out
+
+
+def cavity3d_local_velocity(res: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,
+                            dtype=np.float64) -> Case:
+    """cavity3dBenchmark in its default configuration (LID_DRIVEN without
+    LID_DRIVEN_BOUNCE_BACK, cavity3d.cpp:111-124): the geometry of cavity3d
+    with LocalVelocity boundaries on the walls (u = 0) and on the lid
+    (u = (u_lid, 0, 0))."""
+    c = cavity3d(res, u_lid=u_lid, re=re, padding=padding, dtype=dtype)
+    omega = c.info["omega"]
+    c.name = f"cavity3d_lv_res{res}"
+    c.dynamics[2] = LocalVelocity(omega=omega)
+    c.dynamics[3] = LocalVelocity(omega=omega, wall_velocity=(u_lid, 0.0, 0.0))
+    return c
 
 
 def cavity3d(res: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,

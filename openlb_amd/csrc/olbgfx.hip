@@ -45,6 +45,7 @@ constexpr int Q = 19;
 // largest workgroup of the step kernel; more waves per block would cap the
 // registers per lane below what the D3Q19 collision needs without spilling
 constexpr uint32_t kMaxBlock = 256;
+constexpr int kMaskedBoundary = -2;  // step-kernel MODE: masked, LocalVelocity kinds compiled in
 // LDS of one gfx950 compute unit
 constexpr uint32_t kLdsPerCu = 160 * 1024;
 // persistent blocks per ensemble launch at most (bounds the ticket atomics)
@@ -143,6 +144,7 @@ struct DynEntry {
   A force[3];
   A wall_u[3];
   A wall_rho;
+  A smago_pre;   // Smagorinsky prefactor C^2 invCs2^2 2 sqrt(2) (collisionLES.h:71-73)
 };
 
 template <typename A>
@@ -167,6 +169,7 @@ struct StepArgs {
   const A* cell_u;           // optional VELOCITY field (3 per cell)
   const A* cell_rho;         // optional RHO field
   const A* cell_force;       // optional FORCE field (3 per cell)
+  const uint8_t* normal;     // optional discrete normal code per cell, 9(nx+1)+3(ny+1)+(nz+1)
   double* stats;             // 4 doubles per block when statistics are on
   DynEntry<A> table[OLBGFX_MAX_DYNAMICS];
 };
@@ -422,6 +425,44 @@ __device__ __forceinline__ bool apply_kind(A (&f)[Q], const DynEntry<A>& e, uint
     st_rho = rho;
     st_usqr = ux * ux + uy * uy + uz * uz;
     return true;
+  } else if constexpr (KIND == OLBGFX_SMAGORINSKY_BGK) {
+    // SmagorinskyEffectiveOmega<BGK> (collisionLES.h:59-89): stress of
+    // lbm.h:334-352 with rho - 1 = drho, norm of lbm.h:555-564
+    A drho, jx, jy, jz;
+    moments(f, drho, jx, jy, jz);
+    const A rho = drho + A(1);
+    const A ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    A sxx = A(0), syy = A(0), szz = A(0), sxy = A(0), sxz = A(0), syz = A(0);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if constexpr (true) {
+        const int cx = CX(i), cy = CY(i), cz = CZ(i);
+        if (cx != 0) sxx += f[i];
+        if (cy != 0) syy += f[i];
+        if (cz != 0) szz += f[i];
+        if (cx * cy > 0) sxy += f[i];
+        if (cx * cy < 0) sxy -= f[i];
+        if (cx * cz > 0) sxz += f[i];
+        if (cx * cz < 0) sxz -= f[i];
+        if (cy * cz > 0) syz += f[i];
+        if (cy * cz < 0) syz -= f[i];
+      }
+    }
+    const A third = A(1) / A(3);
+    const A pxx = sxx - rho * ux * ux - third * drho;
+    const A pyy = syy - rho * uy * uy - third * drho;
+    const A pzz = szz - rho * uz * uz - third * drho;
+    const A pxy = sxy - rho * ux * uy;
+    const A pxz = sxz - rho * ux * uz;
+    const A pyz = syz - rho * uy * uz;
+    const A norm2 = pxx * pxx + pyy * pyy + pzz * pzz + A(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
+    const A tau_mol = A(1) / e.omega;
+    const A tau_turb = A(0.5) * (sqrt(tau_mol * tau_mol + e.smago_pre / rho * sqrt(norm2)) - tau_mol);
+    const A om = A(1) / (tau_mol + tau_turb);
+    relax<false>(f, drho, rho, ux, uy, uz, om, om);
+    st_rho = rho;
+    st_usqr = ux * ux + uy * uy + uz * uz;
+    return true;
   } else if constexpr (KIND == OLBGFX_BOUNCE_BACK) {
     revert(f, false, A(0), A(0), A(0), A(0));
     return false;
@@ -459,9 +500,143 @@ __device__ __forceinline__ bool apply_any(A (&f)[Q], const DynEntry<A>& e, uint3
     return apply_kind<OLBGFX_BOUNCE_BACK>(f, e, n, a, st_rho, st_usqr);
   case OLBGFX_BOUNCE_BACK_VELOCITY:
     return apply_kind<OLBGFX_BOUNCE_BACK_VELOCITY>(f, e, n, a, st_rho, st_usqr);
+  case OLBGFX_SMAGORINSKY_BGK:
+    return apply_kind<OLBGFX_SMAGORINSKY_BGK>(f, e, n, a, st_rho, st_usqr);
   default:
     return false;
   }
+}
+
+// ---------------------------------------------------------------------------
+// LocalVelocity boundaries: generic loops over the populations (boundary
+// cells are few, so clarity wins over pairing here)
+
+__host__ __device__ constexpr int CA(int i, int axis)
+{
+  return axis == 0 ? CX(i) : (axis == 1 ? CY(i) : CZ(i));
+}
+
+__device__ __forceinline__ int normal_component(uint32_t code, int axis)
+{
+  const int v = axis == 0 ? (int)(code / 9) : (axis == 1 ? (int)(code / 3 % 3) : (int)(code % 3));
+  return v - 1;
+}
+
+template <typename A>
+__device__ __forceinline__ A wall_component(const StepArgs<A>& a, const DynEntry<A>& e, uint32_t n, int axis)
+{
+  return a.cell_u ? a.cell_u[3 * (size_t)n + axis] : e.wall_u[axis];
+}
+
+// shifted second-order equilibrium of population i (lbm.h:51-64)
+template <typename A>
+__device__ __forceinline__ A feq_i(int i, A rho, const A (&u)[3], A usqr)
+{
+  const A cu = A(CX(i)) * u[0] + A(CY(i)) * u[1] + A(CZ(i)) * u[2];
+  const A t = A(1) / A(TDEN(i));
+  return rho * t * (A(1) + A(3) * cu + A(4.5) * cu * cu - A(1.5) * usqr) - t;
+}
+
+// velocityBMRho (momenta/elements.h:55-71): density implied by the known
+// populations of a fixed-velocity face with the given direction/orientation
+template <typename A>
+__device__ __forceinline__ A velocity_bm_rho(const A (&f)[Q], int dir, int orient, A un)
+{
+  A on_wall = A(0), normal = A(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int c = CA(i, dir);
+    if (c == 0) {
+      on_wall += f[i];
+    } else if (c == orient) {
+      normal += f[i];
+    }
+  }
+  return (A(2) * normal + on_wall + A(1)) / (A(1) + A(orient) * un);
+}
+
+// fromPiToFneq (lbm.h:248-270); pi = (xx, xy, xz, yy, yz, zz)
+template <typename A>
+__device__ __forceinline__ A pi_to_fneq(int i, const A (&pi)[6])
+{
+  const A cx = A(CX(i)), cy = A(CY(i)), cz = A(CZ(i));
+  const A third = A(1) / A(3);
+  A r = (cx * cx - third) * pi[0] + A(2) * cx * cy * pi[1] + A(2) * cx * cz * pi[2] +
+        (cy * cy - third) * pi[3] + A(2) * cy * cz * pi[4] + (cz * cz - third) * pi[5];
+  return r * (A(4.5) / A(TDEN(i)));
+}
+
+// CombinedRLBdynamics<BGK, RegularizedVelocityBoundaryTuple<dir, orient>>::collide
+// (dynamics.h:506-517, boundary/helper.h:31-81): regularise around the
+// boundary momenta, then BGK with the same momenta
+template <typename A>
+__device__ __forceinline__ void local_velocity(A (&f)[Q], int dir, int orient, const A (&u)[3], A om,
+                                               A& rho_out)
+{
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  A rho = velocity_bm_rho(f, dir, orient, u[dir]);
+  A pi[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const int c = CA(i, dir);
+    if (c == 0 || c == orient) {
+      const A w = (c == 0 ? A(1) : A(2)) * (f[i] - feq_i(i, rho, u, usqr));
+      const A cx = A(CX(i)), cy = A(CY(i)), cz = A(CZ(i));
+      pi[0] += cx * cx * w;
+      pi[1] += cx * cy * w;
+      pi[2] += cx * cz * w;
+      pi[3] += cy * cy * w;
+      pi[4] += cy * cz * w;
+      pi[5] += cz * cz * w;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi);
+  }
+  rho = velocity_bm_rho(f, dir, orient, u[dir]);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = f[i] + om * (feq_i(i, rho, u, usqr) - f[i]);
+  }
+  rho_out = rho;
+}
+
+// LOCAL_VELOCITY and FIXED_VELOCITY cells (the masked kernel with boundary
+// kinds); returns true: both yield a statistic
+template <typename A>
+__device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
+                                                        const StepArgs<A>& a, A& st_rho, A& st_usqr)
+{
+  const A u[3] = {wall_component(a, e, n, 0), wall_component(a, e, n, 1), wall_component(a, e, n, 2)};
+  A rho;
+  if (e.kind == OLBGFX_LOCAL_VELOCITY) {
+    const uint32_t code = a.normal[n];
+    int dir = 0;
+    int orient = normal_component(code, 0);
+    if (normal_component(code, 1) != 0) {
+      dir = 1;
+      orient = normal_component(code, 1);
+    } else if (normal_component(code, 2) != 0) {
+      dir = 2;
+      orient = normal_component(code, 2);
+    }
+    local_velocity(f, dir, orient, u, e.omega, rho);
+  } else {
+    // BulkDensity + FixedVelocityMomentumGeneric, BGK (collision.h:57-80)
+    A s = A(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s += f[i];
+    rho = s + A(1);
+    const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      f[i] = f[i] + e.omega * (feq_i(i, rho, u, usqr) - f[i]);
+    }
+  }
+  st_rho = rho;
+  st_usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -570,6 +745,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
 
   if constexpr (MODE >= 0) {
     counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
+  } else if constexpr (MODE == kMaskedBoundary) {
+    const DynEntry<A> e = a.table[id];
+    if (kind >= OLBGFX_LOCAL_VELOCITY) {
+      counted = apply_velocity_boundary(f, e, n, a, st_rho, st_usqr);
+    } else {
+      counted = apply_any(f, e, n, a, st_rho, st_usqr);
+    }
   } else {
     const DynEntry<A> e = a.table[id];
     counted = apply_any(f, e, n, a, st_rho, st_usqr);
@@ -595,9 +777,145 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
 }
 
 // ---------------------------------------------------------------------------
+// PostStream processors of LocalVelocity edges and corners
+// (boundary/boundaryPostProcessors3D.hh:164-294), run after the step on the
+// streamed populations: a.shift holds the offsets after the rotation
+
+template <typename S>
+__device__ __forceinline__ void load_cell(const StepArgs<typename Storage<S>::A>& a, uint32_t n,
+                                          typename Storage<S>::A (&f)[Q])
+{
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = Storage<S>::load(ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)));
+  }
+}
+
+// computeRho of cell m through its momenta
+template <typename S>
+__device__ typename Storage<S>::A momenta_rho(const StepArgs<typename Storage<S>::A>& a, uint32_t m)
+{
+  using A = typename Storage<S>::A;
+  A f[Q];
+  load_cell<S>(a, m, f);
+  const DynEntry<A>& e = a.table[a.mask[m]];
+  if (e.kind == OLBGFX_LOCAL_VELOCITY) {
+    const uint32_t code = a.normal[m];
+    for (int ax = 0; ax < 3; ++ax) {
+      const int o = normal_component(code, ax);
+      if (o != 0) {
+        return velocity_bm_rho(f, ax, o, wall_component(a, e, m, ax));
+      }
+    }
+  }
+  A s = A(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) s += f[i];
+  return s + A(1);
+}
+
+// computeU of cell m through its momenta
+template <typename S>
+__device__ void momenta_u(const StepArgs<typename Storage<S>::A>& a, uint32_t m, typename Storage<S>::A (&u)[3])
+{
+  using A = typename Storage<S>::A;
+  const DynEntry<A>& e = a.table[a.mask[m]];
+  if (e.kind == OLBGFX_LOCAL_VELOCITY || e.kind == OLBGFX_FIXED_VELOCITY) {
+    for (int ax = 0; ax < 3; ++ax) u[ax] = wall_component(a, e, m, ax);
+    return;
+  }
+  A f[Q];
+  load_cell<S>(a, m, f);
+  A drho, jx, jy, jz;
+  moments(f, drho, jx, jy, jz);
+  const A rho = drho + A(1);
+  u[0] = jx / rho;
+  u[1] = jy / rho;
+  u[2] = jz / rho;
+  if (e.kind == OLBGFX_FORCED_BGK || e.kind == OLBGFX_FORCED_TRT) {
+    for (int ax = 0; ax < 3; ++ax) {
+      u[ax] += (a.cell_force ? a.cell_force[3 * (size_t)m + ax] : e.force[ax]) * A(0.5);
+    }
+  }
+}
+
+// DirectedGradients3D along the boundary normal (finiteDifference3D.h:132-155):
+// -orient * (-3 u0 + 4 u1 - u2) / 2 with u1, u2 one and two cells inwards
+template <typename S>
+__device__ void one_sided(const StepArgs<typename Storage<S>::A>& a, uint32_t n, int64_t step, int orient,
+                          typename Storage<S>::A (&d)[3])
+{
+  using A = typename Storage<S>::A;
+  A u0[3], u1[3], u2[3];
+  momenta_u<S>(a, n, u0);
+  momenta_u<S>(a, (uint32_t)((int64_t)n - orient * step), u1);
+  momenta_u<S>(a, (uint32_t)((int64_t)n - 2 * orient * step), u2);
+  for (int b = 0; b < 3; ++b) {
+    d[b] = A(-orient) * ((-A(3) * u0[b] + A(4) * u1[b] - A(1) * u2[b]) / A(2));
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_velocity_post(const StepArgs<typename Storage<S>::A> a,
+                                                       const uint32_t* cells, uint32_t ncells)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) {
+    return;
+  }
+  const uint32_t n = cells[t];
+  const uint32_t code = a.normal[n];
+  const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
+  const int64_t step[3] = {(int64_t)a.py * a.pz, (int64_t)a.pz, 1};
+  const DynEntry<A>& e = a.table[a.mask[n]];
+  A rho;
+  A du[3][3];
+  if (N[0] == 0 || N[1] == 0 || N[2] == 0) {
+    // OuterVelocityEdgeProcessor3D<plane, normal1, normal2>
+    const int p = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+    const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+    const int n1 = N[d1], n2 = N[d2];
+    const A rho10 = momenta_rho<S>(a, (uint32_t)((int64_t)n - n1 * step[d1]));
+    const A rho01 = momenta_rho<S>(a, (uint32_t)((int64_t)n - n2 * step[d2]));
+    const A rho20 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * n1 * step[d1]));
+    const A rho02 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * n2 * step[d2]));
+    rho = A(2) / A(3) * (rho01 + rho10) - A(1) / A(6) * (rho02 + rho20);
+    A up[3], um[3];
+    momenta_u<S>(a, (uint32_t)((int64_t)n + step[p]), up);
+    momenta_u<S>(a, (uint32_t)((int64_t)n - step[p]), um);
+    for (int b = 0; b < 3; ++b) du[p][b] = (up[b] - um[b]) / A(2);
+    one_sided<S>(a, n, step[d1], n1, du[d1]);
+    one_sided<S>(a, n, step[d2], n2, du[d2]);
+  } else {
+    // OuterVelocityCornerProcessor3D<xNormal, yNormal, zNormal>
+    const A rho100 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[0] * step[0]));
+    const A rho010 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[1] * step[1]));
+    const A rho001 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[2] * step[2]));
+    const A rho200 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[0] * step[0]));
+    const A rho020 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[1] * step[1]));
+    const A rho002 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[2] * step[2]));
+    rho = A(4) / A(9) * (rho001 + rho010 + rho100) - A(1) / A(9) * (rho002 + rho020 + rho200);
+    for (int ax = 0; ax < 3; ++ax) one_sided<S>(a, n, step[ax], N[ax], du[ax]);
+  }
+  // regularised populations (boundaryPostProcessors3D.hh:196-215)
+  const A s_to_pi = -rho / A(3) / e.omega;
+  const A pi[6] = {A(2) * du[0][0] * s_to_pi, (du[0][1] + du[1][0]) * s_to_pi, (du[0][2] + du[2][0]) * s_to_pi,
+                   A(2) * du[1][1] * s_to_pi, (du[1][2] + du[2][1]) * s_to_pi, A(2) * du[2][2] * s_to_pi};
+  const A u[3] = {wall_component(a, e, n, 0), wall_component(a, e, n, 1), wall_component(a, e, n, 2)};
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    *ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)) =
+        Storage<S>::pack(feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // the step kernel of one lattice
 //
-// MODE < 0: every cell reads its dynamics id from the mask;
+// MODE = -1: every cell reads its dynamics id from the mask;
+// MODE = -2: the same with the LocalVelocity kinds compiled in;
 // MODE = k: every core cell has the dynamics of table slot uniform_id, whose
 //           kind is k, so the mask is not read and only that path is compiled.
 
@@ -947,6 +1265,14 @@ struct olbgfx_lattice {
   uint32_t block_threads = 256;
   uint32_t lds_reserve = 0;    // dynamic LDS per block that limits occupancy
   int32_t xcd_run_log2 = -1;   // block order of the step kernel (olbgfx_set_block_order)
+  // LocalVelocity boundaries: discrete normals, and the cells whose
+  // post-processors run after each step (edges first, then corners)
+  std::vector<int8_t> host_normals;  // 3 per cell, empty = none
+  uint8_t* normal_dev = nullptr;     // code 9(nx+1)+3(ny+1)+(nz+1) per cell
+  uint32_t* post_dev = nullptr;
+  uint32_t post_edges = 0, post_corners = 0;
+  bool boundary_kinds = false;       // some core cell has a LocalVelocity kind
+  uint64_t boundary_version = ~0ull;
   olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
   std::vector<uint8_t> host_mask;
   uint64_t version = 0;        // bumped by every change of dynamics, mask or fields
@@ -975,6 +1301,10 @@ void fill_table(const olbgfx_lattice* L, DynEntry<A>* t)
       t[k].wall_u[ax] = (A)s.wall_velocity[ax];
     }
     t[k].wall_rho = (A)s.wall_rho;
+    // collisionLES.h:71-73: C^2 * invCs2 * invCs2 * 2 in the value type, times
+    // sqrt(2) in double
+    const A c = (A)s.smagorinsky;
+    t[k].smago_pre = (A)((double)(c * c * (A)3 * (A)3 * (A)2) * std::sqrt(2.0));
   }
 }
 
@@ -1010,6 +1340,7 @@ olbgfx_status fill_args(olbgfx_lattice* L, StepArgs<A>& a, bool device_shifts)
   a.cell_u = (const A*)L->cell_u;
   a.cell_rho = (const A*)L->cell_rho;
   a.cell_force = (const A*)L->cell_force;
+  a.normal = L->normal_dev;
   a.stats = L->stats_partial;
   fill_table(L, a.table);
   return OLBGFX_OK;
@@ -1034,16 +1365,27 @@ void launch_mode(int mode, dim3 grid, dim3 block, size_t lds, hipStream_t stream
   case OLBGFX_FORCED_TRT:
     step_kernel<S, OLBGFX_FORCED_TRT, STATS><<<grid, block, lds, stream>>>(a);
     break;
+  case OLBGFX_SMAGORINSKY_BGK:
+    step_kernel<S, OLBGFX_SMAGORINSKY_BGK, STATS><<<grid, block, lds, stream>>>(a);
+    break;
+  case kMaskedBoundary:
+    step_kernel<S, kMaskedBoundary, STATS><<<grid, block, lds, stream>>>(a);
+    break;
   default:
     step_kernel<S, -1, STATS><<<grid, block, lds, stream>>>(a);
     break;
   }
 }
 
+olbgfx_status prepare_boundary(olbgfx_lattice* L);
+
 template <typename S>
 olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream)
 {
   using A = typename Storage<S>::A;
+  if (prepare_boundary(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
   StepArgs<A> a;
   fill_args(L, a, false);
   if (a.ncore == 0) {
@@ -1060,7 +1402,10 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
   }
   const dim3 grid(8 * a.blocks_per_xcd);
   const dim3 block(L->block_threads);
-  const int mode = L->uniform_id >= 0 ? L->table[L->uniform_id].kind : -1;
+  int mode = L->uniform_id >= 0 ? L->table[L->uniform_id].kind : -1;
+  if (L->boundary_kinds) {
+    mode = kMaskedBoundary;
+  }
   for (uint32_t k = 0; k < steps; ++k) {
     for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
     if (L->stats_on) {
@@ -1074,6 +1419,18 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
     for (int i = 0; i < Q; ++i) {
       const uint64_t s = (uint64_t)L->shift[i] + (uint64_t)L->dmod[i];
       L->shift[i] = (uint32_t)(s >= L->count ? s - L->count : s);
+    }
+    // PostStream: LocalVelocity edges (priority 0), then corners (priority 1)
+    if (L->post_edges + L->post_corners > 0) {
+      for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
+      if (L->post_edges) {
+        k_velocity_post<S><<<(L->post_edges + 255) / 256, 256, 0, stream>>>(a, L->post_dev, L->post_edges);
+      }
+      if (L->post_corners) {
+        k_velocity_post<S><<<(L->post_corners + 255) / 256, 256, 0, stream>>>(a, L->post_dev + L->post_edges,
+                                                                             L->post_corners);
+      }
+      HIP_TRY(hipGetLastError());
     }
   }
   if (steps > 0) {
@@ -1207,6 +1564,71 @@ void recompute_uniform(olbgfx_lattice* L)
   L->uniform_id = id < 0 ? -1 : id;
 }
 
+// find the LocalVelocity cells and the post-processed edges and corners;
+// checks that every LOCAL_VELOCITY cell has a flat normal and that the
+// processors' neighbours (up to two cells inwards) lie in the padded block
+olbgfx_status prepare_boundary(olbgfx_lattice* L)
+{
+  if (L->boundary_version == L->version) {
+    return OLBGFX_OK;
+  }
+  bool any = false;
+  for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
+    any = any || L->table[k].kind >= OLBGFX_LOCAL_VELOCITY;
+  }
+  std::vector<uint32_t> edges, corners;
+  bool used = false;
+  const olbgfx_lattice_desc& D = L->desc;
+  const int P[3] = {D.nx + 2 * D.padding, D.ny + 2 * D.padding, D.nz + 2 * D.padding};
+  for (int x = 0; any && x < D.nx; ++x) {
+    for (int y = 0; y < D.ny; ++y) {
+      for (int z = 0; z < D.nz; ++z) {
+        const int c[3] = {x + D.padding, y + D.padding, z + D.padding};
+        const uint32_t n = ((uint32_t)c[0] * L->py + (uint32_t)c[1]) * L->pz + (uint32_t)c[2];
+        const int kind = L->table[L->host_mask[n]].kind;
+        if (kind < OLBGFX_LOCAL_VELOCITY) {
+          continue;
+        }
+        used = true;
+        int nz = 0;
+        int N[3] = {0, 0, 0};
+        if (!L->host_normals.empty()) {
+          for (int a = 0; a < 3; ++a) {
+            N[a] = L->host_normals[3 * (size_t)n + a];
+            nz += N[a] != 0;
+          }
+        }
+        if (kind == OLBGFX_LOCAL_VELOCITY && nz != 1) {
+          return fail(OLBGFX_ERR_INVALID, "a LOCAL_VELOCITY cell needs a flat discrete normal");
+        }
+        if (kind == OLBGFX_FIXED_VELOCITY && nz >= 2) {
+          for (int a = 0; a < 3; ++a) {
+            const int reach = N[a] == 0 ? (nz == 2 ? 1 : 0) : 2;
+            if (c[a] - reach < 0 || c[a] + reach >= P[a]) {
+              return fail(OLBGFX_ERR_INVALID, "an edge or corner processor reaches outside the block");
+            }
+          }
+          (nz == 2 ? edges : corners).push_back(n);
+        }
+      }
+    }
+  }
+  if (L->post_dev) {
+    HIP_TRY(hipFree(L->post_dev));
+    L->post_dev = nullptr;
+  }
+  L->post_edges = (uint32_t)edges.size();
+  L->post_corners = (uint32_t)corners.size();
+  if (!edges.empty() || !corners.empty()) {
+    edges.insert(edges.end(), corners.begin(), corners.end());
+    HIP_TRY(hipMalloc(&L->post_dev, sizeof(uint32_t) * edges.size()));
+    HIP_TRY(hipMemcpy(L->post_dev, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice));
+  }
+  L->boundary_kinds = used;
+  L->boundary_version = L->version;
+  return OLBGFX_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1308,6 +1730,8 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->cell_u);
   hipFree(L->cell_rho);
   hipFree(L->cell_force);
+  hipFree(L->normal_dev);
+  hipFree(L->post_dev);
   hipFree(L->stats_partial);
   hipFree(L->stats_out);
   delete L;
@@ -1353,7 +1777,7 @@ olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_d
   if (id >= OLBGFX_MAX_DYNAMICS) {
     return fail(OLBGFX_ERR_INVALID, "dynamics id out of range");
   }
-  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_BOUNCE_BACK_VELOCITY) {
+  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_SMAGORINSKY_BGK) {
     return fail(OLBGFX_ERR_UNSUPPORTED, "dynamics kind not provided by the gfx950 platform");
   }
   L->table[id] = *dyn;
@@ -1379,6 +1803,41 @@ olbgfx_status olbgfx_set_dynamics_mask(olbgfx_lattice* L, const uint8_t* ids)
   std::memcpy(L->host_mask.data(), ids, L->count);
   HIP_TRY(hipMemcpy(L->mask, ids, L->count, hipMemcpyHostToDevice));
   recompute_uniform(L);
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* L, const int8_t* normals)
+{
+  if (L) {
+    ++L->version;
+  }
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (!normals) {
+    L->host_normals.clear();
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(L->normal_dev));
+    L->normal_dev = nullptr;
+    return OLBGFX_OK;
+  }
+  std::vector<uint8_t> codes(L->count);
+  for (uint32_t n = 0; n < L->count; ++n) {
+    int code = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int v = normals[3 * (size_t)n + a];
+      if (v < -1 || v > 1) {
+        return fail(OLBGFX_ERR_INVALID, "discrete normal entries must be -1, 0 or 1");
+      }
+      code = 3 * code + (v + 1);
+    }
+    codes[n] = (uint8_t)code;
+  }
+  L->host_normals.assign(normals, normals + 3 * (size_t)L->count);
+  if (!L->normal_dev) {
+    HIP_TRY(hipMalloc(&L->normal_dev, L->count));
+  }
+  HIP_TRY(hipMemcpy(L->normal_dev, codes.data(), L->count, hipMemcpyHostToDevice));
   return OLBGFX_OK;
 }
 
@@ -1740,6 +2199,14 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
 {
   using A = typename Storage<S>::A;
   const size_t m = E->members.size();
+  for (olbgfx_lattice* L : E->members) {
+    if (prepare_boundary(L) != OLBGFX_OK) {
+      return OLBGFX_ERR_INVALID;
+    }
+    if (L->boundary_kinds) {
+      return fail(OLBGFX_ERR_UNSUPPORTED, "ensembles do not take members with LocalVelocity boundaries");
+    }
+  }
   bool changed = force || E->tiled_block != E->block_threads || E->grouped_bytes != E->group_bytes;
   for (size_t k = 0; k < m; ++k) {
     changed = changed || E->versions[k] != E->members[k]->version;
@@ -2018,8 +2485,8 @@ olbgfx_status olbgfx_ensemble_step(olbgfx_ensemble* E, uint32_t steps, void* str
   hipStream_t s = (hipStream_t)stream;
   olbgfx_status st = dispatch_ensemble(E, [&](auto* tag) -> olbgfx_status {
     using S = std::remove_pointer_t<decltype(tag)>;
-    if (ensemble_prepare<S>(E, false) != OLBGFX_OK) {
-      return OLBGFX_ERR_HIP;
+    if (const olbgfx_status p = ensemble_prepare<S>(E, false); p != OLBGFX_OK) {
+      return p;
     }
     return ensemble_enqueue<S>(E, steps, s);
   });
@@ -2059,8 +2526,8 @@ olbgfx_status olbgfx_ensemble_capture(olbgfx_ensemble* E, uint32_t steps_per_lau
   }
   olbgfx_status st = dispatch_ensemble(E, [&](auto* tag) -> olbgfx_status {
     using S = std::remove_pointer_t<decltype(tag)>;
-    if (ensemble_prepare<S>(E, false) != OLBGFX_OK) {
-      return OLBGFX_ERR_HIP;
+    if (const olbgfx_status p = ensemble_prepare<S>(E, false); p != OLBGFX_OK) {
+      return p;
     }
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipStreamBeginCapture(E->capture_stream, hipStreamCaptureModeThreadLocal));

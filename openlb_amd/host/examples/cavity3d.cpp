@@ -8,6 +8,7 @@
  *
  *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
  *            [--STORAGE f32|f16|bf16] [--DUMP file] [--TUNE_CACHE file]
+ *            [--BOUNDARY local_velocity|bounce_back]
  *
  * --DUMP writes the materials and the populations of the padded block after
  * the run (logical order, float32) for the tests.
@@ -33,6 +34,7 @@ struct Parameters {
   Storage storage = Storage::F32;
   std::string dump;
   std::string tuneCache;  // "" = no autotuning
+  bool localVelocity = true;  // cavity3d.cpp:111-124: LID_DRIVEN without LID_DRIVEN_BOUNCE_BACK
 };
 
 static Parameters fromCLI(int argc, char** argv)
@@ -46,6 +48,12 @@ static Parameters fromCLI(int argc, char** argv)
     else if (k == "--REYNOLDS") p.reynolds = std::stof(v);
     else if (k == "--DUMP") p.dump = v;
     else if (k == "--TUNE_CACHE") p.tuneCache = v;
+    else if (k == "--BOUNDARY") {
+      if (v != "local_velocity" && v != "bounce_back") {
+        throw std::invalid_argument("--BOUNDARY is local_velocity or bounce_back");
+      }
+      p.localVelocity = v == "local_velocity";
+    }
     else if (k == "--STORAGE") {
       p.storage = v == "f16" ? Storage::F16 : (v == "bf16" ? Storage::BF16 : Storage::F32);
     } else {
@@ -70,8 +78,13 @@ static void prepareLattice(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T
                            const Parameters& p)
 {
   dynamics::set<BGKdynamics>(lattice, geometry, 1);
-  boundary::set<boundary::BounceBack>(lattice, geometry, 2);
-  dynamics::set<BounceBackVelocity>(lattice, geometry, 3);
+  if (p.localVelocity) {
+    boundary::set<boundary::LocalVelocity>(lattice, geometry, 2);
+    boundary::set<boundary::LocalVelocity>(lattice, geometry, 3);
+  } else {
+    boundary::set<boundary::BounceBack>(lattice, geometry, 2);
+    dynamics::set<BounceBackVelocity>(lattice, geometry, 3);
+  }
   lattice.setStatisticsOff();
 }
 

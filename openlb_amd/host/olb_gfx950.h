@@ -5,7 +5,8 @@
  *   SuperGeometry::rename                      geometry/superGeometry.h, blockGeometry.hh:401-466
  *   SuperLattice::defineDynamics<DYNAMICS>     core/superLattice.h:228-250
  *   dynamics::set / boundary::set              case/setters.h:303-345, boundary/setBoundary.h:150-157
- *   SuperLattice::setParameter<OMEGA / MAGIC>  core/superLattice.h (setParameter)
+ *   boundary::set<boundary::LocalVelocity>     boundary/localVelocity3D.h:36-120
+ *   SuperLattice::setParameter<OMEGA / MAGIC / LES::SMAGORINSKY>  core/superLattice.h (setParameter)
  *   SuperLattice::defineField<FORCE>, defineU  core/superLattice.h:255-330
  *   SuperLattice::iniEquilibrium               core/blockLattice.hh:202-236
  *   SuperLattice::initialize, collideAndStream core/superLattice.hh:248-266, 663-725
@@ -34,6 +35,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "olbgfx.h"
@@ -87,6 +89,9 @@ namespace collision {
 struct TRT {
   struct MAGIC { };
 };
+namespace LES {
+struct SMAGORINSKY { };  // dynamics/collisionLES.h:37
+}  // namespace LES
 }  // namespace collision
 
 // ---- dynamics (dynamics/dynamics.h) --------------------------------------
@@ -110,6 +115,9 @@ template <typename T, typename DESCRIPTOR> struct BounceBack {
 };
 template <typename T, typename DESCRIPTOR> struct BounceBackVelocity {
   static constexpr int32_t kind = OLBGFX_BOUNCE_BACK_VELOCITY;
+};
+template <typename T, typename DESCRIPTOR> struct SmagorinskyBGKdynamics {
+  static constexpr int32_t kind = OLBGFX_SMAGORINSKY_BGK;  // dynamics/smagorinskyBGKdynamics.h:37-44
 };
 
 /// population storage of the platform
@@ -150,6 +158,8 @@ public:
     return (CellID)(((std::size_t)(x + _overlap) * _size[1] + (y + _overlap)) * _size[2] + (z + _overlap));
   }
   int get(int x, int y, int z) const { return _material[getCellId(x, y, z)]; }
+  /// material, or 0 beyond the padded block
+  int getOrOutside(int x, int y, int z) const { return getClamped(x, y, z); }
   void set(int x, int y, int z, int m) { _material[getCellId(x, y, z)] = (std::uint8_t)m; }
 
   /// every core cell of material fromM becomes toM (blockGeometry.hh:401-408)
@@ -231,6 +241,80 @@ struct LatticeStatistics {
   T getMaxU() const { return maxU; }
 };
 
+// ---- discrete normals (geometry/discreteNormals.h:33-57) ------------------
+enum class NormalType { None, Flat, ExternalEdge, ExternalCorner };
+
+
+/// type and outward normal of a boundary cell by the neighbourhood rules
+/// of computeBoundaryTypeAndNormal (geometry/discreteNormals.hh:176-1190)
+/// for box walls; B = neither fluid (1) nor outside (0):
+///  flat: the four in-plane axis neighbours are B, one neighbour along the
+///   remaining axis is fluid, the normal points away from it;
+///  external edge along p: neighbours +-e_p are B, transverse s_a e_a and
+///   s_b e_b are B, +-e_p + s_b e_b are not fluid, s_a e_a + s_b e_b is
+///   fluid; normal -(s_a e_a + s_b e_b);
+///  external corner: s_x e_x, s_y e_y, s_z e_z and their pairwise sums are
+///   B and s is fluid; normal -s.
+/// A cell that matches none or several gives None.
+template <typename T>
+std::pair<NormalType, std::array<int, 3>> discreteNormal(const SuperGeometry<T>& g, int x, int y, int z)
+{
+  auto m = [&](std::array<int, 3> d) { return g.getOrOutside(x + d[0], y + d[1], z + d[2]); };
+  auto fluid = [&](std::array<int, 3> d) { return m(d) == 1; };
+  auto bnd = [&](std::array<int, 3> d) { const int v = m(d); return v != 1 && v != 0; };
+  auto unit = [](int a, int s) { std::array<int, 3> e{0, 0, 0}; e[a] = s; return e; };
+  auto add = [](std::array<int, 3> u, std::array<int, 3> v) {
+    return std::array<int, 3>{u[0] + v[0], u[1] + v[1], u[2] + v[2]};
+  };
+  std::vector<std::pair<NormalType, std::array<int, 3>>> found;
+  auto note = [&](NormalType t, std::array<int, 3> n) {
+    for (const auto& f : found) {
+      if (f.first == t && f.second == n) return;
+    }
+    found.emplace_back(t, n);
+  };
+  for (int a = 0; a < 3; ++a) {
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    if (bnd(unit(b, 1)) && bnd(unit(b, -1)) && bnd(unit(c, 1)) && bnd(unit(c, -1))) {
+      for (int s : {1, -1}) {
+        if (fluid(unit(a, s))) note(NormalType::Flat, unit(a, -s));
+      }
+    }
+  }
+  for (int p = 0; p < 3; ++p) {
+    if (!bnd(unit(p, 1)) || !bnd(unit(p, -1))) continue;
+    const int ax = (p + 1) % 3, bx = (p + 2) % 3;
+    for (int sa : {1, -1}) {
+      for (int sb : {1, -1}) {
+        const std::array<std::array<int, 3>, 2> pairs[2] = {{unit(ax, sa), unit(bx, sb)},
+                                                            {unit(bx, sb), unit(ax, sa)}};
+        for (const auto& pr : pairs) {
+          const auto ea = pr[0], eb = pr[1];
+          if (bnd(eb) && !fluid(add(unit(p, 1), eb)) && !fluid(add(unit(p, -1), eb)) && bnd(ea) &&
+              fluid(add(ea, eb))) {
+            const auto d = add(ea, eb);
+            note(NormalType::ExternalEdge, {-d[0], -d[1], -d[2]});
+          }
+        }
+      }
+    }
+  }
+  for (int sx : {1, -1}) {
+    for (int sy : {1, -1}) {
+      for (int sz : {1, -1}) {
+        const auto ex = unit(0, sx), ey = unit(1, sy), ez = unit(2, sz);
+        if (bnd(ex) && bnd(ey) && bnd(ez) && bnd(add(ex, ey)) && bnd(add(ex, ez)) && bnd(add(ey, ez)) &&
+            fluid({sx, sy, sz})) {
+          note(NormalType::ExternalCorner, {-sx, -sy, -sz});
+        }
+      }
+    }
+  }
+  if (found.size() != 1) return {NormalType::None, {0, 0, 0}};
+  return found[0];
+}
+
+
 // ---- the lattice ---------------------------------------------------------
 
 template <typename T, typename DESCRIPTOR = descriptors::D3Q19>
@@ -278,6 +362,36 @@ public:
     _dirty = true;
   }
 
+  /// LocalVelocity boundary on the cells of `material`
+  /// (boundary::set<boundary::LocalVelocity<T,D,BGKdynamics>>,
+  /// boundary/setBoundary.h:76-100, boundary/localVelocity3D.h:46-115): each
+  /// cell gets its discrete normal (fluid = material 1, outside = material
+  /// 0); flat faces become regularised velocity cells, external edges and
+  /// corners fixed-velocity BGK cells with their post-processors.  Other
+  /// configurations (internal edges and corners) throw std::domain_error.
+  void setLocalVelocity(SuperGeometry<T>& geometry, int material)
+  {
+    const int flat = slotFor(OLBGFX_LOCAL_VELOCITY);
+    const int fixed = slotFor(OLBGFX_FIXED_VELOCITY);
+    if (_normals.empty()) {
+      _normals.assign(3 * _ncells, 0);
+    }
+    geometry.forCore([&](int x, int y, int z) {
+      if (geometry.get(x, y, z) != material) return;
+      const auto [type, n] = discreteNormal(geometry, x, y, z);
+      const CellID c = geometry.getCellId(x, y, z);
+      if (type == NormalType::Flat) {
+        _slotOfCell[c] = (std::uint8_t)flat;
+      } else if (type == NormalType::ExternalEdge || type == NormalType::ExternalCorner) {
+        _slotOfCell[c] = (std::uint8_t)fixed;
+      } else {
+        throw std::domain_error("LocalVelocity on this platform covers flat faces, external edges and corners");
+      }
+      for (int a = 0; a < 3; ++a) _normals[3 * (std::size_t)c + a] = (std::int8_t)n[a];
+    });
+    _dirty = true;
+  }
+
   /// set a dynamics parameter on every dynamics that has it
   template <typename PARAMETER>
   void setParameter(T value)
@@ -291,6 +405,10 @@ public:
       } else if constexpr (std::is_same_v<PARAMETER, collision::TRT::MAGIC>) {
         if (s.kind == OLBGFX_TRT || s.kind == OLBGFX_FORCED_TRT) {
           s.magic = value;
+        }
+      } else if constexpr (std::is_same_v<PARAMETER, collision::LES::SMAGORINSKY>) {
+        if (s.kind == OLBGFX_SMAGORINSKY_BGK) {
+          s.smagorinsky = value;
         }
       } else {
         static_assert(sizeof(PARAMETER) == 0, "parameter not provided by the gfx950 platform");
@@ -350,6 +468,7 @@ public:
       check(olbgfx_set_dynamics(handle(), (uint32_t)k, &descs[k]));
     }
     check(olbgfx_set_dynamics_mask(handle(), _slotOfCell.data()));
+    check(olbgfx_set_cell_normals(handle(), _normals.empty() ? nullptr : _normals.data()));
     for (auto& kv : _fields) {
       check(olbgfx_set_cell_field(handle(), kv.first, valueType(),
                                   kv.second.uniform ? nullptr : kv.second.values.data()));
@@ -442,12 +561,14 @@ private:
     int32_t kind;
     T omega = 0;
     T magic = 0;
+    T smagorinsky = 0;
     olbgfx_dynamics_desc desc() const
     {
       olbgfx_dynamics_desc d{};
       d.kind = kind;
       d.omega = (double)omega;
       d.magic = (double)magic;
+      d.smagorinsky = (double)smagorinsky;
       d.wall_rho = 1.0;
       return d;
     }
@@ -550,6 +671,7 @@ private:
   std::size_t _ncells = 0;
   std::vector<Slot> _slots;
   std::vector<std::uint8_t> _slotOfCell;
+  std::vector<std::int8_t> _normals;  // discrete normal per cell for LocalVelocity
   std::map<int32_t, Field> _fields;
   std::vector<T> _host;
   bool _hostValid = false;
@@ -572,10 +694,17 @@ struct BounceBack {
   template <typename T, typename DESCRIPTOR>
   using dynamics = olb::BounceBack<T, DESCRIPTOR>;
 };
+/// boundary::LocalVelocity<T,D,BGKdynamics> (boundary/localVelocity3D.h:36-120)
+struct LocalVelocity {
+};
 template <typename BC, typename T, typename DESCRIPTOR>
 void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material)
 {
-  lattice.template defineDynamics<BC::template dynamics>(geometry, material);
+  if constexpr (std::is_same_v<BC, LocalVelocity>) {
+    lattice.setLocalVelocity(geometry, material);
+  } else {
+    lattice.template defineDynamics<BC::template dynamics>(geometry, material);
+  }
 }
 }  // namespace boundary
 

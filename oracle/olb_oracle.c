@@ -43,6 +43,19 @@
  *     RHO (FixedDensity) and VELOCITY fields:
  *       src/dynamics/collision.h:494-520, src/dynamics/dynamics.h:634-640,
  *       src/dynamics/momenta/elements.h:270-297, 797-830
+ *   - LocalVelocity boundaries (boundary/localVelocity3D.h:36-120): flat
+ *     faces collide as CombinedRLBdynamics<BGK, RegularizedVelocityBoundaryTuple>
+ *     (dynamics/dynamics.h:476-526; density momenta/elements.h:55-71, 358-365;
+ *     stress boundary/helper.h:31-81; fromPiToFneq dynamics/lbm.h:248-270),
+ *     edges and corners as BGK with FixedVelocityBoundaryTuple
+ *     (momenta/aliases.h:215-220) followed after streaming by
+ *     OuterVelocityEdgeProcessor3D (priority 0) and
+ *     OuterVelocityCornerProcessor3D (priority 1)
+ *     (boundary/boundaryPostProcessors3D.hh:164-294; finite differences
+ *     utilities/finiteDifference.h:35-45, finiteDifference3D.h:72-191)
+ *   - Smagorinsky BGK (SmagorinskyBGKdynamics, dynamics/smagorinskyBGKdynamics.h:37-44):
+ *     effective omega of dynamics/collisionLES.h:59-89 from the stress of
+ *     dynamics/lbm.h:334-352 and its norm dynamics/lbm.h:555-564
  *   - lattice statistics (cells whose collision returns rho != -1 count):
  *       src/dynamics/interface.h:44-52, src/core/latticeStatistics.h:43-70
  *
@@ -76,6 +89,9 @@ enum {
   KIND_TRT_GUO = 4,
   KIND_BOUNCE_BACK = 5,
   KIND_BOUNCE_BACK_VELOCITY = 6,
+  KIND_LOCAL_VELOCITY = 7,   /* flat LocalVelocity face: CombinedRLB<BGK, RegularizedVelocityBoundaryTuple> */
+  KIND_FIXED_VELOCITY = 8,   /* LocalVelocity edge/corner: BGK<FixedVelocityBoundaryTuple> + post-processor */
+  KIND_SMAGORINSKY_BGK = 9,  /* SmagorinskyBGKdynamics */
 };
 
 enum { STORE_NATIVE = 0, STORE_F16 = 1, STORE_BF16 = 2 };
@@ -102,6 +118,7 @@ typedef struct {
   real force[3];
   real wall_u[3];
   real wall_rho;
+  real smagorinsky;
 } dyn_t;
 
 typedef struct {
@@ -118,6 +135,7 @@ typedef struct {
   real* cell_velocity;  /* optional per-cell VELOCITY (count*3) */
   real* cell_rho;       /* optional per-cell RHO (count) */
   real* cell_force;     /* optional per-cell FORCE (count*3) */
+  int8_t* normals;      /* optional per-cell discrete normal (count*3) */
   int store;
   int stats_on;
   double stats[4];      /* nCells, sum rho, sum uSqr, max uSqr (last step) */
@@ -265,6 +283,7 @@ void ORC(destroy)(void* h)
   free(L->cell_velocity);
   free(L->cell_rho);
   free(L->cell_force);
+  free(L->normals);
   free(L);
 }
 
@@ -305,7 +324,7 @@ void ORC(set_dynamics_mask)(void* h, const uint8_t* ids)
 }
 
 void ORC(set_dynamics)(void* h, int id, int kind, double omega, double magic,
-                       const double* force, const double* wall_u, double wall_rho)
+                       const double* force, const double* wall_u, double wall_rho, double smagorinsky)
 {
   lattice_t* L = (lattice_t*)h;
   dyn_t* D = &L->table[id & 255];
@@ -317,6 +336,7 @@ void ORC(set_dynamics)(void* h, int id, int kind, double omega, double magic,
     D->wall_u[a] = wall_u ? (real)wall_u[a] : (real)0;
   }
   D->wall_rho = (real)wall_rho;
+  D->smagorinsky = (real)smagorinsky;
 }
 
 static void set_field(real** dst, uint64_t n, const real* src)
@@ -458,6 +478,79 @@ static void revert(real f[Q])
   }
 }
 
+/* ---- LocalVelocity boundaries ---- */
+
+static const real* wall_velocity(const lattice_t* L, const dyn_t* D, uint64_t n)
+{
+  return L->cell_velocity ? &L->cell_velocity[3 * n] : D->wall_u;
+}
+
+/* direction and orientation of a flat discrete normal (one non-zero entry) */
+static int flat_normal(const lattice_t* L, uint64_t n, int* dir, int* orient)
+{
+  if (!L->normals) return 0;
+  int k = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (L->normals[3 * n + a] != 0) {
+      *dir = a;
+      *orient = L->normals[3 * n + a];
+      ++k;
+    }
+  }
+  return k == 1;
+}
+
+/* momenta/elements.h:55-71 velocityBMRho */
+static real velocity_bm_rho(const real f[Q], int dir, int orient, const real u[3])
+{
+  real on_wall = 0, normal = 0;
+  for (int i = 0; i < Q; ++i) if (C[i][dir] == 0) on_wall += f[i];
+  for (int i = 0; i < Q; ++i) if (C[i][dir] == orient) normal += f[i];
+  return ((real)2 * normal + on_wall + (real)1) / ((real)1 + (real)orient * u[dir]);
+}
+
+/* boundary/helper.h:31-81 BoundaryHelpers::computeStress; pi in the order
+ * xx, xy, xz, yy, yz, zz */
+static void boundary_stress(const real f[Q], int dir, int orient, real rho, const real u[3], real pi[6])
+{
+  const real usqr = norm_sqr(u);
+  real fneq[Q];
+  for (int i = 0; i < Q; ++i) {
+    fneq[i] = (i > 0 && (C[i][dir] == 0 || C[i][dir] == orient))
+                ? f[i] - eq_second_order(i, rho, u, usqr) : (real)0;
+  }
+  int k = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = a; b < 3; ++b) {
+      real v = 0;
+      for (int i = 0; i < Q; ++i) if (C[i][dir] == 0) v += (real)(C[i][a] * C[i][b]) * fneq[i];
+      for (int i = 0; i < Q; ++i) if (C[i][dir] == orient) v += (real)2 * (real)(C[i][a] * C[i][b]) * fneq[i];
+      pi[k++] = v;
+    }
+  }
+}
+
+/* lbm.h:248-270 fromPiToFneq */
+static real pi_to_fneq(int i, const real pi[6])
+{
+  real fneq = 0;
+  int k = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = a; b < 3; ++b) {
+      real add = (real)(C[i][a] * C[i][b]);
+      if (a == b) {
+        add -= (real)1 / (real)3;
+      } else {
+        add *= (real)2;
+      }
+      add *= pi[k++];
+      fneq += add;
+    }
+  }
+  fneq *= weight(i) * (real)3 * (real)3 / (real)2;
+  return fneq;
+}
+
 /* one cell; returns 1 if the cell contributes to the statistics */
 static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho, real* st_usqr)
 {
@@ -501,6 +594,58 @@ static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho,
       }
     }
     *st_rho = r; *st_usqr = norm_sqr(uw);
+    return 1;
+  }
+  case KIND_SMAGORINSKY_BGK: {
+    /* SmagorinskyEffectiveOmega<BGK> (collisionLES.h:59-89) with the stress
+     * of lbm.h:334-352 and the norm of lbm.h:555-564 */
+    rho_u(f, &rho, u);
+    real pi[6];
+    int k = 0;
+    for (int a = 0; a < 3; ++a) {
+      for (int b = a; b < 3; ++b) {
+        real v = 0;
+        for (int i = 0; i < Q; ++i) v += (real)(C[i][a] * C[i][b]) * f[i];
+        v -= rho * u[a] * u[b];
+        if (a == b) v -= (real)1 / (real)3 * (rho - (real)1);
+        pi[k++] = v;
+      }
+    }
+    real norm_sqr_pi = pi[0] * pi[0] + (real)(2. * pi[1] * pi[1]) + pi[2] * pi[2];
+    norm_sqr_pi += pi[2] * pi[2] + pi[3] * pi[3] + (real)(2. * pi[4] * pi[4]) + pi[5] * pi[5];
+    const real c = D->smagorinsky;
+    const real pre = (real)((double)(c * c * (real)3 * (real)3 * (real)2) * sqrt(2.0));
+    const real tau_mol = (real)1 / D->omega;
+    const real tau_turb = (real)0.5 * ((real)sqrt(tau_mol * tau_mol + pre / rho * (real)sqrt(norm_sqr_pi)) - tau_mol);
+    const real omega_eff = (real)1 / (tau_mol + tau_turb);
+    bgk(f, rho, u, omega_eff);
+    *st_rho = rho; *st_usqr = norm_sqr(u);
+    return 1;
+  }
+  case KIND_LOCAL_VELOCITY: {
+    /* CombinedRLBdynamics::collide (dynamics.h:506-517): regularise around
+     * (rho, u, pi) of the boundary momenta, then BGK with the same momenta */
+    int dir = 0, orient = 0;
+    if (!flat_normal(L, n, &dir, &orient)) return 0;
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    real pi[6];
+    rho = velocity_bm_rho(f, dir, orient, uu);
+    boundary_stress(f, dir, orient, rho, uu, pi);
+    const real usqr = norm_sqr(uu);
+    for (int i = 0; i < Q; ++i) f[i] = eq_second_order(i, rho, uu, usqr) + pi_to_fneq(i, pi);
+    rho = velocity_bm_rho(f, dir, orient, uu);
+    bgk(f, rho, uu, D->omega);
+    *st_rho = rho; *st_usqr = usqr;
+    return 1;
+  }
+  case KIND_FIXED_VELOCITY: {
+    /* BulkDensity + FixedVelocityMomentumGeneric, BGK */
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    rho_u(f, &rho, u);
+    bgk(f, rho, uu, D->omega);
+    *st_rho = rho; *st_usqr = norm_sqr(uu);
     return 1;
   }
   default:
@@ -583,6 +728,157 @@ static void stream(lattice_t* L)
   }
 }
 
+/* ---- PostStream processors of LocalVelocity edges and corners ---- */
+
+static void load_cell(const lattice_t* L, uint64_t n, real f[Q])
+{
+  for (int i = 0; i < Q; ++i) f[i] = L->ring[i][slot(L, i, n)];
+}
+
+/* computeRho / computeU of a cell through its momenta */
+static real momenta_rho(const lattice_t* L, uint64_t n)
+{
+  const dyn_t* D = &L->table[L->mask[n]];
+  real f[Q], rho, u[3];
+  load_cell(L, n, f);
+  if (D->kind == KIND_LOCAL_VELOCITY) {
+    int dir = 0, orient = 0;
+    flat_normal(L, n, &dir, &orient);
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    return velocity_bm_rho(f, dir, orient, uu);
+  }
+  rho_u(f, &rho, u);
+  return rho;
+}
+
+static void momenta_u(const lattice_t* L, uint64_t n, real u[3])
+{
+  const dyn_t* D = &L->table[L->mask[n]];
+  if (D->kind == KIND_LOCAL_VELOCITY || D->kind == KIND_FIXED_VELOCITY) {
+    const real* uw = wall_velocity(L, D, n);
+    for (int a = 0; a < 3; ++a) u[a] = uw[a];
+    return;
+  }
+  real f[Q], rho;
+  load_cell(L, n, f);
+  rho_u(f, &rho, u);
+  if (D->kind == KIND_BGK_GUO || D->kind == KIND_TRT_GUO) {
+    const real* F = L->cell_force ? &L->cell_force[3 * n] : D->force;
+    for (int a = 0; a < 3; ++a) u[a] += F[a] * (real)0.5;
+  }
+}
+
+static uint64_t step_cell(const lattice_t* L, uint64_t n, int axis, int k)
+{
+  return (uint64_t)((int64_t)n + (int64_t)k * L->proj[axis]);
+}
+
+/* finiteDifference.h:42-45 */
+static real boundary_gradient(real u0, real u1, real u2)
+{
+  return (-(real)3 * u0 + (real)4 * u1 - (real)1 * u2) / (real)2;
+}
+
+/* DirectedGradients3D<..., orthogonal = true>: one-sided along axis with the
+ * outward orientation (finiteDifference3D.h:132-155) */
+static void one_sided(const lattice_t* L, uint64_t n, int axis, int orient, real d[3])
+{
+  real u0[3], u1[3], u2[3];
+  momenta_u(L, n, u0);
+  momenta_u(L, step_cell(L, n, axis, -orient), u1);
+  momenta_u(L, step_cell(L, n, axis, -2 * orient), u2);
+  for (int a = 0; a < 3; ++a) d[a] = (real)(-orient) * boundary_gradient(u0[a], u1[a], u2[a]);
+}
+
+/* regularised populations from rho, the cell's velocity and the velocity
+ * gradients dA_uB[A][B] = d_A u_B (boundaryPostProcessors3D.hh:196-215) */
+static void regularise(lattice_t* L, uint64_t n, real rho, real du[3][3])
+{
+  const dyn_t* D = &L->table[L->mask[n]];
+  const real s_to_pi = -rho / (real)3 / D->omega;
+  real pi[6];
+  pi[0] = (real)2 * du[0][0] * s_to_pi;
+  pi[3] = (real)2 * du[1][1] * s_to_pi;
+  pi[5] = (real)2 * du[2][2] * s_to_pi;
+  pi[1] = (du[0][1] + du[1][0]) * s_to_pi;
+  pi[2] = (du[0][2] + du[2][0]) * s_to_pi;
+  pi[4] = (du[1][2] + du[2][1]) * s_to_pi;
+  real u[3];
+  momenta_u(L, n, u);
+  const real usqr = norm_sqr(u);
+  for (int i = 0; i < Q; ++i) {
+    L->ring[i][slot(L, i, n)] = orc_quantize(L, eq_second_order(i, rho, u, usqr) + pi_to_fneq(i, pi));
+  }
+}
+
+/* OuterVelocityEdgeProcessor3D<plane, normal1, normal2>::apply */
+static void post_edge(lattice_t* L, uint64_t n, const int N[3])
+{
+  const int p = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+  const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+  const int n1 = N[d1], n2 = N[d2];
+  const real rho10 = momenta_rho(L, step_cell(L, n, d1, -n1));
+  const real rho01 = momenta_rho(L, step_cell(L, n, d2, -n2));
+  const real rho20 = momenta_rho(L, step_cell(L, n, d1, -2 * n1));
+  const real rho02 = momenta_rho(L, step_cell(L, n, d2, -2 * n2));
+  const real rho = (real)2 / (real)3 * (rho01 + rho10) - (real)1 / (real)6 * (rho02 + rho20);
+  real du[3][3], up[3], um[3];
+  momenta_u(L, step_cell(L, n, p, 1), up);
+  momenta_u(L, step_cell(L, n, p, -1), um);
+  for (int b = 0; b < 3; ++b) du[p][b] = (up[b] - um[b]) / (real)2;
+  one_sided(L, n, d1, n1, du[d1]);
+  one_sided(L, n, d2, n2, du[d2]);
+  regularise(L, n, rho, du);
+}
+
+/* OuterVelocityCornerProcessor3D<xNormal, yNormal, zNormal>::apply */
+static void post_corner(lattice_t* L, uint64_t n, const int N[3])
+{
+  const real rho100 = momenta_rho(L, step_cell(L, n, 0, -N[0]));
+  const real rho010 = momenta_rho(L, step_cell(L, n, 1, -N[1]));
+  const real rho001 = momenta_rho(L, step_cell(L, n, 2, -N[2]));
+  const real rho200 = momenta_rho(L, step_cell(L, n, 0, -2 * N[0]));
+  const real rho020 = momenta_rho(L, step_cell(L, n, 1, -2 * N[1]));
+  const real rho002 = momenta_rho(L, step_cell(L, n, 2, -2 * N[2]));
+  const real rho = (real)4 / (real)9 * (rho001 + rho010 + rho100)
+                 - (real)1 / (real)9 * (rho002 + rho020 + rho200);
+  real du[3][3];
+  for (int a = 0; a < 3; ++a) one_sided(L, n, a, N[a], du[a]);
+  regularise(L, n, rho, du);
+}
+
+/* PostStream stage: edges (priority 0) before corners (priority 1) */
+static void post_stream(lattice_t* L)
+{
+  if (!L->normals) return;
+  for (int pass = 2; pass <= 3; ++pass) {
+    for (int x = 0; x < L->nx; ++x) {
+      for (int y = 0; y < L->ny; ++y) {
+        for (int z = 0; z < L->nz; ++z) {
+          const uint64_t n = (uint64_t)((x + L->pad) * L->proj[0] + (y + L->pad) * L->proj[1] + (z + L->pad));
+          if (L->table[L->mask[n]].kind != KIND_FIXED_VELOCITY) continue;
+          const int N[3] = {L->normals[3 * n], L->normals[3 * n + 1], L->normals[3 * n + 2]};
+          const int nz = (N[0] != 0) + (N[1] != 0) + (N[2] != 0);
+          if (nz != pass) continue;
+          if (pass == 2) post_edge(L, n, N); else post_corner(L, n, N);
+        }
+      }
+    }
+  }
+}
+
+void ORC(set_cell_normals)(void* h, const int8_t* normals)
+{
+  lattice_t* L = (lattice_t*)h;
+  free(L->normals);
+  L->normals = NULL;
+  if (normals) {
+    L->normals = (int8_t*)malloc(3 * L->count);
+    memcpy(L->normals, normals, 3 * L->count);
+  }
+}
+
 void ORC(collide_and_stream)(void* h, int steps)
 {
   lattice_t* L = (lattice_t*)h;
@@ -590,6 +886,7 @@ void ORC(collide_and_stream)(void* h, int steps)
     collide(L);
     periodic_copy(L);
     stream(L);
+    post_stream(L);
     L->time += 1;
   }
 }

@@ -45,10 +45,11 @@ def library() -> ctypes.CDLL:
             g("get_statistics").argtypes = [vp, vp]
             g("set_dynamics_mask").argtypes = [vp, vp]
             g("set_dynamics").argtypes = [vp, i32, i32, ctypes.c_double, ctypes.c_double, vp, vp,
-                                          ctypes.c_double]
+                                          ctypes.c_double, ctypes.c_double]
             g("set_cell_velocity").argtypes = [vp, vp]
             g("set_cell_rho").argtypes = [vp, vp]
             g("set_cell_force").argtypes = [vp, vp]
+            g("set_cell_normals").argtypes = [vp, vp]
             g("get_populations").argtypes = [vp, vp]
             g("set_populations").argtypes = [vp, vp]
             g("collide_and_stream").argtypes = [vp, i32]
@@ -133,7 +134,7 @@ class OracleLattice:
         wall = np.asarray(dynamics.wall_velocity, dtype=np.float64)
         self._f("set_dynamics")(self._h, int(slot), int(dynamics.kind), float(dynamics.omega),
                                 float(dynamics.magic), _ptr(force), _ptr(wall),
-                                float(dynamics.wall_rho))
+                                float(dynamics.wall_rho), float(getattr(dynamics, "smagorinsky", 0.0)))
 
     def set_dynamics_mask(self, ids) -> None:
         ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)
@@ -147,6 +148,15 @@ class OracleLattice:
             return
         v = np.ascontiguousarray(values, dtype=self._dtype).reshape(-1)
         self._f(name)(self._h, _ptr(v))
+
+    def set_cell_normals(self, normals) -> None:
+        """discrete normal per cell (3 x int8, CellID order), None removes them"""
+        if normals is None:
+            self._f("set_cell_normals")(self._h, None)
+            return
+        v = np.ascontiguousarray(normals, dtype=np.int8).reshape(-1)
+        assert v.size == 3 * self.ncells
+        self._f("set_cell_normals")(self._h, _ptr(v))
 
     def ini_equilibrium(self, rho, u, sel=None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)

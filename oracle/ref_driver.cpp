@@ -12,7 +12,9 @@
  * Usage (REF_VALUE = float or double decides the value type):
  *   olb_ref cavity   RES STEPS ULID RE OUT        cavity3dBenchmark geometry, BGK +
  *                                                BounceBack walls + BounceBackVelocity lid
+ *   olb_ref cavity_lv RES STEPS ULID RE OUT       the same with LocalVelocity walls and lid
  *   olb_ref tgv      N STEPS U0 TAU OUT           periodic Taylor-Green, BGK
+ *   olb_ref tgv_smagorinsky N STEPS U0 TAU CS OUT  the same with SmagorinskyBGKdynamics
  *   olb_ref porous   NX NY NZ SPHERES R TAU MAGIC FX STEPS OUT
  *                                                periodic sphere pack, ForcedTRT + BounceBack
  *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
@@ -116,7 +118,42 @@ UnitConverter<T,DESCRIPTOR> lattice_units(T nu)
 }
 
 // ---- lid-driven cavity: the geometry of examples/laminar/cavity3dBenchmark ----
-int run_cavity(int res, int steps, double ulid, double re, const std::string& out)
+// discrete normals of the boundary materials as boundary::set computes them
+// (boundary/setBoundary.h:76-100, geometry/discreteNormals.hh:176): type per
+// padded cell (-1 where not a boundary cell) and the normal, CellID order
+void dump_normals(Dump& d, SuperGeometry<T,3>& G, int padding, std::initializer_list<int> boundary)
+{
+  auto fluid = G.getMaterialIndicator(1);
+  auto outside = G.getMaterialIndicator(0);
+  auto& fl = fluid->getBlockIndicatorF(0);
+  auto& os = outside->getBlockIndicatorF(0);
+  auto& bg = G.getBlockGeometry(0);
+  std::vector<int32_t> type, normal;
+  const int nx = bg.getNx(), ny = bg.getNy(), nz = bg.getNz();
+  for (int x = -padding; x < nx + padding; ++x) {
+    for (int y = -padding; y < ny + padding; ++y) {
+      for (int z = -padding; z < nz + padding; ++z) {
+        const int m = bg.get({x, y, z});
+        bool on = false;
+        for (int b : boundary) {
+          on = on || m == b;
+        }
+        if (on && bg.getNeighborhoodRadius({x, y, z}) >= 2) {
+          const auto [t, n] = computeBoundaryTypeAndNormal<T>(fl, os, {x, y, z});
+          type.push_back((int32_t)t);
+          normal.insert(normal.end(), {n[0], n[1], n[2]});
+        } else {
+          type.push_back(-1);
+          normal.insert(normal.end(), {0, 0, 0});
+        }
+      }
+    }
+  }
+  d.array("normal_type", "i4", type);
+  d.array("normals", "i4", normal);
+}
+
+int run_cavity(int res, int steps, double ulid, double re, const std::string& out, bool local_velocity)
 {
   using DESCRIPTOR = descriptors::D3Q19<>;
   const T length = 1;
@@ -143,18 +180,28 @@ int run_cavity(int res, int steps, double ulid, double re, const std::string& ou
   SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), G);
   L.setStatisticsOff();
   dynamics::set<BGKdynamics>(L, G, 1);
-  boundary::set<boundary::BounceBack>(L, G, 2);
-  dynamics::set<BounceBackVelocity>(L, G, 3);
+  if (local_velocity) {
+    // the default of cavity3dBenchmark (LID_DRIVEN without LID_DRIVEN_BOUNCE_BACK)
+    boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR>>>(L, G, 2);
+    boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR>>>(L, G, 3);
+  } else {
+    boundary::set<boundary::BounceBack>(L, G, 2);
+    dynamics::set<BounceBackVelocity>(L, G, 3);
+  }
   AnalyticalConst3D<T,T> uLid(T(ulid), T(0), T(0));
   L.defineU(G.getMaterialIndicator(3), uLid);
   const T omega = T(1) / (T(3) * nu + T(0.5));
   L.setParameter<descriptors::OMEGA>(omega);
   L.initialize();
 
-  Dump d(out, block_header(L) + " case=cavity res=" + std::to_string(res) + " steps=" +
-                  std::to_string(steps) + " omega=" + std::to_string((double)omega) +
-                  " ulid=" + std::to_string(ulid) + " nu=" + std::to_string((double)nu));
+  Dump d(out, block_header(L) + (local_velocity ? " case=cavity_lv" : " case=cavity") + " res=" +
+                  std::to_string(res) + " steps=" + std::to_string(steps) + " omega=" +
+                  std::to_string((double)omega) + " ulid=" + std::to_string(ulid) + " nu=" +
+                  std::to_string((double)nu));
   d.array("materials", "i4", materials(G, 3));
+  if (local_velocity) {
+    dump_normals(d, G, 3, {2, 3});
+  }
   d.array("distances", "i8", distances(L));
   d.array("f0", tname(), populations(L));
   for (int t = 0; t < steps; ++t) {
@@ -197,7 +244,7 @@ void build_periodic_box(int nx, int ny, int nz, CuboidDecomposition3D<T>*& cuboi
   G->communicate();
 }
 
-int run_tgv(int n, int steps, double u0, double tau, const std::string& out, bool bench)
+int run_tgv(int n, int steps, double u0, double tau, const std::string& out, bool bench, double smagorinsky = -1)
 {
   using DESCRIPTOR = descriptors::D3Q19<>;
   CuboidDecomposition3D<T>* cuboids;
@@ -207,7 +254,13 @@ int run_tgv(int n, int steps, double u0, double tau, const std::string& out, boo
   const T nu = T((tau - 0.5) / 3.0);
   SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), *G);
   L.setStatisticsOff();
-  dynamics::set<BGKdynamics>(L, *G, 1);
+  if (smagorinsky >= 0) {
+    // the bulk dynamics of examples/turbulence/tgv3d (tgv3d.cpp:143, 165)
+    dynamics::set<SmagorinskyBGKdynamics>(L, *G, 1);
+    L.setParameter<collision::LES::SMAGORINSKY>(T(smagorinsky));
+  } else {
+    dynamics::set<BGKdynamics>(L, *G, 1);
+  }
   L.setParameter<descriptors::OMEGA>(T(1.0 / tau));
   TgvField rho(u0, n, true), u(u0, n, false);
   L.iniEquilibrium(*G, 1, rho, u);
@@ -225,7 +278,8 @@ int run_tgv(int n, int steps, double u0, double tau, const std::string& out, boo
     return 0;
   }
   Dump d(out, block_header(L) + " case=tgv n=" + std::to_string(n) + " steps=" + std::to_string(steps) +
-                  " u0=" + std::to_string(u0) + " tau=" + std::to_string(tau));
+                  " u0=" + std::to_string(u0) + " tau=" + std::to_string(tau) +
+                  (smagorinsky >= 0 ? " smagorinsky=" + std::to_string(smagorinsky) : std::string()));
   d.array("materials", "i4", materials(*G, 3));
   d.array("distances", "i8", distances(L));
   d.array("f0", tname(), populations(L));
@@ -409,11 +463,16 @@ int main(int argc, char** argv)
   }
   const std::string mode = argv[1];
   try {
-    if (mode == "cavity" && argc == 7) {
-      return run_cavity(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6]);
+    if ((mode == "cavity" || mode == "cavity_lv") && argc == 7) {
+      return run_cavity(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6],
+                        mode == "cavity_lv");
     }
     if (mode == "tgv" && argc == 7) {
       return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6], false);
+    }
+    if (mode == "tgv_smagorinsky" && argc == 8) {
+      return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[7], false,
+                     std::stod(argv[6]));
     }
     if (mode == "bench_tgv" && argc == 4) {
       return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), 0.04, 0.56, "", true);

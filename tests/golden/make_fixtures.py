@@ -3,7 +3,8 @@ path (oracle/_ref, built by `make -C oracle ref` from the OpenLB sources).
 
 Run in the build container (the OpenLB sources are not on the GPU host):
     python tests/golden/make_fixtures.py            # the small fixtures
-    python tests/golden/make_fixtures.py NAME ...   # full-size ones (FULL)
+    python tests/golden/make_fixtures.py NAME ...   # full-size ones (FULL), or
+                                                    # local_velocity, smagorinsky
 Each fixture holds the materials of the padded block (CellID order), the
 neighbour distances, the populations at t = 0 and after the run for the core
 cells (logical order, shifted by -t_i), and the run's parameters.
@@ -61,18 +62,42 @@ def save(name: str, meta: dict, arrays: dict, extra: dict | None = None, cells=N
     for k in ("energy", "usol"):
         if k in arrays:
             out[k] = arrays[k]
+    if "normal_type" in arrays:
+        out["normal_type"] = np.asarray(arrays["normal_type"]).astype(np.int8)
+        out["normals"] = np.asarray(arrays["normals"]).astype(np.int8)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print("wrote", name, {k: v.shape for k, v in out.items() if hasattr(v, "shape")})
 
 
-def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000) -> None:
-    """config 1 at full size: the cavity3dBenchmark geometry at RESOLUTION 64,
-    BounceBack walls, BounceBackVelocity lid u = 0.1, Re = 100 (tau = 0.692),
-    1000 steps from rest.  Too large to keep whole, so the fixture holds the
+def local_velocity_fixtures() -> None:
+    """cavity3dBenchmark's default boundaries (LocalVelocity on walls and
+    lid) at RESOLUTION 12, 200 steps, FP32 and FP64, plus the reference's
+    discrete normals of the boundary cells"""
+    for tag, binary in (("f32", "olb_ref_f"), ("f64", "olb_ref_d")):
+        path = os.path.join(RUN, f"cavity_lv_{tag}.bin")
+        run(binary, ["cavity_lv", "12", "200", "0.1", "100", path])
+        save(f"cavity3d_lv_res12_{tag}", *read_dump(path))
+
+
+def smagorinsky_fixtures() -> None:
+    """periodic Taylor-Green 12^3 with the bulk dynamics of tgv3d
+    (SmagorinskyBGKdynamics, C = 0.1), tau = 0.505 so the eddy viscosity
+    dominates, 100 steps, FP32 and FP64"""
+    for tag, binary in (("f32", "olb_ref_f"), ("f64", "olb_ref_d")):
+        path = os.path.join(RUN, f"tgv_smagorinsky_{tag}.bin")
+        run(binary, ["tgv_smagorinsky", "12", "100", "0.04", "0.505", "0.1", path])
+        save(f"tgv12_smagorinsky_{tag}", *read_dump(path))
+
+
+def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode: str = "cavity") -> None:
+    """the cavity3dBenchmark geometry at RESOLUTION 64, Re = 100 (tau = 0.692),
+    lid u = 0.1, 1000 steps from rest; mode "cavity": BounceBack walls and a
+    BounceBackVelocity lid, mode "cavity_lv": LocalVelocity walls and lid.
+    Too large to keep whole, so the fixture holds the
     velocity on the mid-plane z = res/2, the populations of 256 seeded fluid
     cells, and FP64 sums over the fluid."""
-    path = os.path.join(RUN, f"cavity{res}_{tag}.bin")
-    run(binary, ["cavity", str(res), str(steps), "0.1", "100", path])
+    path = os.path.join(RUN, f"{mode}{res}_{tag}.bin")
+    run(binary, [mode, str(res), str(steps), "0.1", "100", path])
     meta, arrays = read_dump(path)
     q = 19
     n = arrays["f1"].size // q
@@ -92,12 +117,13 @@ def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000) -> N
     sample = np.sort(rng.choice(fl, 256, replace=False))
     uf = u.reshape(-1, 3)[fluid.reshape(-1)]
     sums = np.array([(rho.reshape(-1)[fluid.reshape(-1)] - 1).sum(), *uf.sum(0), (uf ** 2).sum()])
-    np.savez_compressed(os.path.join(HERE, f"cavity3d_res{res}_{steps}steps_{tag}.npz"),
+    name = f"cavity3d{'_lv' if mode == 'cavity_lv' else ''}_res{res}_{steps}steps_{tag}"
+    np.savez_compressed(os.path.join(HERE, name + ".npz"),
                         meta=np.array(json.dumps(meta)), materials=arrays["materials"].astype(np.uint8),
                         u_midplane=mid, sample_cells=sample.astype(np.int64),
                         f1_sample=f[:, sample].astype(np.float32 if tag == "f32" else np.float64),
                         fluid_sums=sums)
-    print("wrote", f"cavity3d_res{res}_{steps}steps_{tag}", sums)
+    print("wrote", name, sums)
 
 
 # full-size workloads, kept as reductions (tests/golden_runs.reduce_state)
@@ -158,8 +184,17 @@ def full_size(name: str) -> None:
 def main() -> None:
     if len(sys.argv) > 1:
         for name in sys.argv[1:]:
-            full_size(name)
+            if name == "local_velocity":
+                local_velocity_fixtures()
+                cavity_reduced("olb_ref_f", "f32", mode="cavity_lv")
+            elif name == "smagorinsky":
+                smagorinsky_fixtures()
+            else:
+                full_size(name)
         return
+    local_velocity_fixtures()
+    smagorinsky_fixtures()
+    cavity_reduced("olb_ref_f", "f32", mode="cavity_lv")
     cavity_reduced("olb_ref_f", "f32")
     for tag, binary in (("f32", "olb_ref_f"), ("f64", "olb_ref_d")):
         path = os.path.join(RUN, f"cavity_{tag}.bin")

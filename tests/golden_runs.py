@@ -7,7 +7,9 @@ import os
 
 import numpy as np
 
-from openlb_amd import BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FORCED_BGK, FORCED_TRT, NO_DYNAMICS, Dynamics
+from openlb_amd import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FORCED_BGK, FORCED_TRT, NO_DYNAMICS,
+                        SMAGORINSKY_BGK, Dynamics)
+from openlb_amd.cases import EXTERNAL_CORNER, EXTERNAL_EDGE, FLAT, LocalVelocity
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -27,6 +29,11 @@ def dynamics_for(meta: dict, storage: str) -> dict:
     """material -> Dynamics of the fixture's case, parameters rounded to the
     value type the way the reference driver forms them"""
     case = meta["case"]
+    if case == "cavity_lv":
+        d = dynamics_for(dict(meta, case="cavity"), storage)
+        om = d[1].omega
+        return {1: d[1], 2: LocalVelocity(omega=om),
+                3: LocalVelocity(omega=om, wall_velocity=(_as_value(meta["ulid"], storage), 0.0, 0.0))}
     if case == "cavity":
         if storage == "f64":
             nu = 0.1 * meta["res"] / 100.0
@@ -39,6 +46,9 @@ def dynamics_for(meta: dict, storage: str) -> dict:
                 3: Dynamics(BOUNCE_BACK_VELOCITY, wall_velocity=(_as_value(meta["ulid"], storage), 0, 0),
                             wall_rho=1.0)}
     if case == "tgv":
+        if "smagorinsky" in meta:
+            return {1: Dynamics(SMAGORINSKY_BGK, omega=_as_value(1.0 / meta["tau"], storage),
+                                smagorinsky=_as_value(meta["smagorinsky"], storage))}
         return {1: Dynamics(BGK, omega=_as_value(1.0 / meta["tau"], storage))}
     if case == "porous":
         return {1: Dynamics(FORCED_TRT, omega=_as_value(1.0 / meta["tau"], storage),
@@ -52,7 +62,7 @@ def dynamics_for(meta: dict, storage: str) -> dict:
 
 
 def periodic_for(meta: dict):
-    return {"cavity": (False,) * 3, "tgv": (True,) * 3, "porous": (True,) * 3,
+    return {"cavity": (False,) * 3, "cavity_lv": (False,) * 3, "tgv": (True,) * 3, "porous": (True,) * 3,
             "poiseuille": (True, False, False)}[meta["case"]]
 
 
@@ -64,10 +74,30 @@ def build(lattice_cls, meta, d, storage: str):
     dyns = dynamics_for(meta, storage)
     lat.set_dynamics(0, Dynamics(NO_DYNAMICS))
     lut = np.zeros(256, dtype=np.uint8)
-    for slot, (mat, dyn) in enumerate(sorted(dyns.items()), start=1):
-        lat.set_dynamics(slot, dyn)
-        lut[mat] = slot
-    lat.set_dynamics_mask(lut[mats])
+    slot = 1
+    lv = {}
+    for mat, dyn in sorted(dyns.items()):
+        if isinstance(dyn, LocalVelocity):
+            from openlb_amd import FIXED_VELOCITY, LOCAL_VELOCITY
+
+            lat.set_dynamics(slot, Dynamics(LOCAL_VELOCITY, omega=dyn.omega, wall_velocity=dyn.wall_velocity))
+            lat.set_dynamics(slot + 1, Dynamics(FIXED_VELOCITY, omega=dyn.omega, wall_velocity=dyn.wall_velocity))
+            lv[mat] = (slot, slot + 1)
+            slot += 2
+        else:
+            lat.set_dynamics(slot, dyn)
+            lut[mat] = slot
+            slot += 1
+    mask = lut[mats]
+    if lv:
+        # the reference's own discrete normals of the boundary cells
+        types = d["normal_type"]
+        for mat, (flat, fixed) in lv.items():
+            on = mats == mat
+            mask[on & (types == FLAT)] = flat
+            mask[on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))] = fixed
+        lat.set_cell_normals(d["normals"])
+    lat.set_dynamics_mask(mask)
     f = np.zeros((19, lat.ncells))
     f[:, d["cells"]] = d["f0"]
     lat.set_populations(f)
@@ -78,7 +108,7 @@ def compared_cells(meta, d) -> np.ndarray:
     """columns of f1 that are compared: the fluid for the closed cavity (its
     walls also hold values streamed in from the padding, which the reference
     keeps in rings of a different length), every core cell otherwise"""
-    if meta["case"] == "cavity":
+    if meta["case"] in ("cavity", "cavity_lv"):
         return np.flatnonzero(d["materials"][d["cells"]] == 1)
     return np.arange(d["cells"].size)
 
@@ -198,3 +228,31 @@ def reduce_state(f, meta: dict, fluid: np.ndarray, sample: np.ndarray, chunk: in
         rho, u = _rho_u(f[:, a:a + chunk][:, sel - a])
         sums += [np.sum(rho - 1.0), *u.sum(1), np.sum(u * u)]
     return up, np.asarray(f[:, sample], dtype=np.float64), sums
+
+
+def local_velocity_boundary_errors(f: np.ndarray, d) -> tuple:
+    """max relative errors of the unshifted populations f (19 x fixture
+    cells) on the LocalVelocity boundary cells against the fixture: flat
+    faces on the populations streamed in from the fluid side (those entering
+    from outside are never read: the regularisation rebuilds them), edges and
+    corners on all 19 (their post-processor rewrites every one)"""
+    from tests.helpers import C, T
+
+    f = np.asarray(f, dtype=np.float64)
+    ref = d["f1"].astype(np.float64)
+    types = d["normal_type"][d["cells"]]
+    normals = d["normals"].reshape(-1, 3)[d["cells"]]
+    flat_err = edge_err = 0.0
+    for k in np.flatnonzero(types >= 0):
+        n = normals[k]
+        if types[k] == 0:
+            a = int(np.flatnonzero(n)[0])
+            rows = [i for i in range(19) if C[i][a] != -n[a]]
+        else:
+            rows = list(range(19))
+        e = float(np.max(np.abs(f[rows, k] - ref[rows, k]) / np.abs(ref[rows, k] + T[rows])))
+        if types[k] == 0:
+            flat_err = max(flat_err, e)
+        else:
+            edge_err = max(edge_err, e)
+    return flat_err, edge_err

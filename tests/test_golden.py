@@ -6,11 +6,16 @@ import pytest
 
 from openlb_amd import cases
 from oracle.oracle import OracleLattice
-from tests.golden_runs import build, compared_cells, core_fluid, load, value_tracer_run, velocity_error_l2
+from tests.golden_runs import (build, compared_cells, core_fluid, load, local_velocity_boundary_errors,
+                               value_tracer_run, velocity_error_l2)
 from tests.helpers import rel_err_unshifted
 
-FIXTURES = ["cavity3d_res12", "tgv12", "porous16x12x12"]
+FIXTURES = ["cavity3d_res12", "cavity3d_lv_res12", "tgv12", "tgv12_smagorinsky", "porous16x12x12"]
 TOL = {"f32": 1e-5, "f64": 1e-12}
+# the reference evaluates Smagorinsky BGK through its generated (CSE) kernel,
+# whose different operation order the non-linear eddy viscosity amplifies to
+# ~1e-9 in FP64 over 100 steps
+TOL_F64 = {"tgv12_smagorinsky": 1e-8}
 
 
 def test_cavity_geometry_bit_exact():
@@ -54,7 +59,7 @@ def test_oracle_matches_reference(name, storage):
     got = lat.get_populations()[:, d["cells"]][:, cols]
     err = rel_err_unshifted(got, d["f1"][:, cols])
     print(name, storage, "max rel err vs reference", err)
-    assert err <= TOL[storage]
+    assert err <= (TOL_F64.get(name, TOL["f64"]) if storage == "f64" else TOL[storage])
 
 
 def test_oracle_poiseuille_known_answer():
@@ -74,7 +79,7 @@ def test_oracle_poiseuille_known_answer():
 
 
 def test_oracle_cavity_res64_1000_steps_matches_reference():
-    """config 1 at full size: cavity3dBenchmark geometry, RESOLUTION 64,
+    """the lid-driven cavity at full size: cavity3dBenchmark geometry, RESOLUTION 64,
     BounceBack + BounceBackVelocity lid u = 0.1, Re = 100, 1000 steps, FP32"""
     from tests.golden_runs import cavity_reduced
 
@@ -110,3 +115,50 @@ def test_porous_geometry_matches_reference_full_size():
     core = np.ascontiguousarray(c.materials[p:-p, p:-p, p:-p])
     np.testing.assert_array_equal(np.bincount(core.reshape(-1), minlength=4), d["material_counts"])
     assert zlib.crc32(core.tobytes()) == int(d["core_materials_crc32"])
+
+
+def test_discrete_normals_match_reference():
+    """the boundary classification of boundary::set (flat faces, external
+    edges and corners of the cavity, outward normals) equals the one the
+    reference computed for its LocalVelocity cavity, cell for cell"""
+    meta, d = load("cavity3d_lv_res12_f32")
+    c = cases.cavity3d_local_velocity(12)
+    np.testing.assert_array_equal(c.materials.reshape(-1), d["materials"])
+    types, normals = c.cell_normals()
+    np.testing.assert_array_equal(types.reshape(-1), d["normal_type"])
+    np.testing.assert_array_equal(normals.reshape(-1), d["normals"])
+    counts = np.bincount(types.reshape(-1)[types.reshape(-1) >= 0], minlength=4)
+    assert list(counts) == [6 * 11 * 11, 8, 0, 12 * 11]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_oracle_local_velocity_boundary_cells(storage):
+    """LocalVelocity walls and lid: the populations the next step reads on
+    the boundary cells themselves"""
+    meta, d = load(f"cavity3d_lv_res12_{storage}")
+    lat = build(OracleLattice, meta, d, storage)
+    lat.collide_and_stream(meta["steps"])
+    flat, edge = local_velocity_boundary_errors(lat.get_populations()[:, d["cells"]], d)
+    print(storage, "flat faces", flat, "edges and corners", edge)
+    assert flat <= TOL[storage] and edge <= TOL[storage]
+
+
+def test_oracle_local_velocity_cavity_res64_1000_steps():
+    """the default cavity3dBenchmark (LocalVelocity walls and lid) at
+    RESOLUTION 64, 1000 steps, FP32, built from the case's own discrete
+    normals: mid-plane velocity within 5e-6 (u_lid = 0.1), sample populations
+    within 1e-5 relative, sum of u^2 over the fluid within 1e-4"""
+    from oracle.oracle import set_threads
+    from tests.golden_runs import cavity_reduced
+
+    set_threads(8)
+    meta, d = load("cavity3d_lv_res64_1000steps_f32")
+    c = cases.cavity3d_local_velocity(64, dtype=np.float32)
+    lat = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(lat)
+    lat.collide_and_stream(meta["steps"])
+    mid, fs, sums = cavity_reduced(lat, meta, d)
+    fluid = c.materials[3:-3, 3:-3, 3 + 32] == 1
+    assert np.max(np.abs(mid - d["u_midplane"])[fluid]) <= 5e-6
+    assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5
+    np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=1e-4)

@@ -31,11 +31,15 @@ def _have(name):
 
 
 # FP32 tolerances against the reference's FP32 run (different summation
-# order and FMA contraction, growing with the step count)
+# order and FMA contraction, growing with the step count).  "u" is an
+# absolute bound on the mid-plane velocity (u0 = 0.04 for the vortex); None
+# bounds it instead by how far the reference's own FP32 velocity lies from an
+# FP64 run of the same workload on the GPU (the porous flow's velocities,
+# ~1e-4, sit five decades below the populations, where FP32 rounding shows)
 TOL = {
-    "tgv256_500steps_f32": {"u": 2e-6, "f": 2e-5, "sums": 2e-5},
-    "porous384x128x128_5000steps_f32": {"u": 2e-8, "f": 2e-5, "sums": 2e-5},
-    "tgv512_1000steps_f32": {"u": 2e-6, "f": 2e-5, "sums": 2e-5},
+    "tgv256_500steps_f32": {"u": 2e-6, "f": 1e-5, "sums": 2e-5},
+    "porous384x128x128_5000steps_f32": {"u": None, "f": 1e-5, "sums": 2e-5},
+    "tgv512_1000steps_f32": {"u": 2e-6, "f": 1e-5, "sums": 2e-5},
 }
 
 
@@ -62,6 +66,21 @@ def test_full_size_fp32_matches_reference(name):
     du = float(np.max(np.abs(up - d["u_plane"])))
     ef = rel_err_unshifted(fs, d["f1_sample"])
     print(f"{name}: max |du| plane {du:.3e}, sample rel {ef:.3e}, sums {sums} ref {d['fluid_sums']}")
-    assert du <= tol["u"]
+    if tol["u"] is None:
+        lat.close()
+        lat64 = Lattice(case.extent, case.padding, case.periodic, "f64")
+        case.apply(lat64)
+        lat64.collide_and_stream(meta["steps"])
+        up64, _, sums64 = reduce_state(lat64.get_populations(), meta, fluid, d["sample_cells"])
+        lat64.close()
+        ref_noise = float(np.max(np.abs(d["u_plane"] - up64)))
+        ours_noise = float(np.max(np.abs(up - up64)))
+        print(f"{name}: reference FP32 vs FP64 {ref_noise:.3e}, ours vs FP64 {ours_noise:.3e}; "
+              f"sum u^2 FP64 {sums64[4]:.9e}")
+        assert du <= 2.0 * ref_noise + 1e-9
+        assert ours_noise <= 2.0 * ref_noise + 1e-9
+        assert abs(sums[4] - sums64[4]) <= 2.0 * abs(d["fluid_sums"][4] - sums64[4]) + 1e-12
+    else:
+        assert du <= tol["u"]
+        np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=tol["sums"])
     assert ef <= tol["f"]
-    np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=tol["sums"])

@@ -17,18 +17,20 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BINARY = os.path.join(ROOT, "openlb_amd", "host", "examples", "cavity3d")
 
 
-def test_cpp_cavity_matches_oracle(tmp_path):
+@pytest.mark.parametrize("boundary", ["bounce_back", "local_velocity"])
+def test_cpp_cavity_matches_oracle(tmp_path, boundary):
     if not os.path.exists(BINARY):
         pytest.fail("openlb_amd/host/examples/cavity3d is not built (run __graft_entry__.build())")
     res, steps = 16, 90
     dump = tmp_path / "cavity.bin"
-    print("[cpp] cavity3d", res, steps, flush=True)
-    out = subprocess.run([BINARY, "--RESOLUTION", str(res), "--TIME_STEPS", str(steps), "--DUMP", str(dump)],
-                         capture_output=True, text=True, timeout=120)
+    print("[cpp] cavity3d", res, steps, boundary, flush=True)
+    out = subprocess.run([BINARY, "--RESOLUTION", str(res), "--TIME_STEPS", str(steps), "--DUMP", str(dump),
+                          "--BOUNDARY", boundary], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     fields = out.stdout.strip().split(",")
     assert int(fields[0]) == res and float(fields[-1]) > 0
-    c = cases.cavity3d(res, dtype=np.float32)
+    make = cases.cavity3d_local_velocity if boundary == "local_velocity" else cases.cavity3d
+    c = make(res, dtype=np.float32)
     raw = np.fromfile(dump, dtype=np.uint8)
     mats = raw[:c.ncells]
     np.testing.assert_array_equal(mats, c.materials.reshape(-1))

@@ -1,0 +1,29 @@
+"""Host-side logic of the C++ mirror that needs no GPU: its discrete-normal
+classification of the cavity walls equals the reference's (the golden
+fixture holds the normals OpenLB's boundary::set computed)."""
+import os
+import subprocess
+
+import numpy as np
+
+from tests.golden_runs import load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_cpp_discrete_normals_match_reference(tmp_path):
+    exe = tmp_path / "normals_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "openlb_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "normals_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), "12"], capture_output=True, text=True, check=True).stdout
+    meta, d = load("cavity3d_lv_res12_f32")
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    types = d["normal_type"].reshape(shape)
+    normals = d["normals"].reshape(shape + (3,))
+    rows = np.array([[int(v) for v in line.split()] for line in out.strip().splitlines()])
+    assert len(rows) == int(np.count_nonzero(types >= 0))
+    for x, y, z, t, nx, ny, nz in rows:
+        assert types[x + p, y + p, z + p] == t
+        assert tuple(normals[x + p, y + p, z + p]) == (nx, ny, nz)
