@@ -27,6 +27,12 @@ HBM_PEAK_GBPS = 8000.0  # MI355X published HBM3E bandwidth
 BYTES_PER_POP = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
 
 
+# with --tune-cache, every autotuner call reads and writes that cache, so a
+# second run with the same file launches exactly the kernels of the first
+# without trial launches (what a kernel trace of the run needs)
+USE_TUNE_CACHE = False
+
+
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +55,8 @@ def parse_args():
     ap.add_argument("--porous-steps", type=int, default=500)
     ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--no-les", action="store_true")
+    ap.add_argument("--tune-cache", default="",
+                    help="autotuner cache file; launch choices found there are reused without trials")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC counter calibration)")
     return ap.parse_args()
@@ -68,7 +76,7 @@ def time_storage(torch, size, storage, steps, warmup, block, tune=True):
     handle = stream.cuda_stream
     tuned = None
     if tune and not block:
-        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=USE_TUNE_CACHE)
         tuned = {k: tuned[k] for k in ("block", "waves", "order", "ms")}
     lat.collide_and_stream(warmup, handle)
     torch.cuda.synchronize()
@@ -125,7 +133,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     handle = torch.cuda.current_stream().cuda_stream
     tuned = None
     if tune:
-        tuned = autotune.tune_ensemble(ens, steps=3, stream=handle, use_cache=False)
+        tuned = autotune.tune_ensemble(ens, steps=3, stream=handle, use_cache=USE_TUNE_CACHE)
         tuned = {k: tuned[k] for k in ("block", "waves", "tiles", "order", "group_bytes", "steps_per_group", "ticket", "ms")}
     ens.step(warmup, handle)
     steps = max(10, steps // 10 * 10)
@@ -134,6 +142,7 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
     ens.launch_graph(1, handle)
     ms_graph = ens.time("graph", steps, handle)
     ms_sep = ens.time("separate", steps, handle)
+    ms_sync = ens.time("separate_synchronized", steps, handle)
     ens.set_schedule(0, 1, False)
     ens.step(2, handle)
     ms_flat = ens.time("persistent", steps, handle)
@@ -150,8 +159,10 @@ def time_ensemble(torch, storage, steps, warmup, tune=True):
         "persistent_glups": cells * steps / (ms_persistent * 1e-3) / 1e9,
         "separate_glups": cells * steps / (ms_sep * 1e-3) / 1e9,
         "ungrouped_persistent_steps_per_s": steps / (ms_flat * 1e-3),
+        "separate_synchronized_steps_per_s": steps / (ms_sync * 1e-3),
     }
     out["speedup_graph_vs_separate"] = ms_sep / ms_graph
+    out["speedup_graph_vs_separate_synchronized"] = ms_sync / ms_graph
     ens.close()
     for lat in lats:
         lat.close()
@@ -163,7 +174,7 @@ def accuracy(torch, size, steps):
     storage from the same analytic start; velocity error of each against the
     FP64 run (relative L2 over the core, max abs relative to u0), and the FP64
     kinetic-energy decay against the analytic rate"""
-    from openlb_amd import Lattice, cases
+    from openlb_amd import Lattice, autotune, cases
 
     n = size
     core = None
@@ -176,6 +187,7 @@ def accuracy(torch, size, steps):
         lat = Lattice(case.extent, case.padding, case.periodic, st)
         case.apply(lat)
         del case
+        autotune.tune_lattice(lat, steps=3, use_cache=USE_TUNE_CACHE)
         e0 = lat.reduce_moments()["sumUSqr"]
         t0 = time.perf_counter()
         lat.collide_and_stream(steps)
@@ -217,7 +229,7 @@ def time_porous(torch, steps, storages=("f32", "f16")):
     for st in storages:
         lat = Lattice(case.extent, case.padding, case.periodic, st)
         case.apply(lat)
-        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=USE_TUNE_CACHE)
         lat.collide_and_stream(10, handle)
         ms = lat.time_steps(steps, handle)
         m = lat.reduce_moments(handle)
@@ -242,7 +254,7 @@ def time_smagorinsky(torch, size, steps, storages=("f32", "f16")):
     for st in storages:
         lat = Lattice(case.extent, case.padding, case.periodic, st)
         case.apply(lat)
-        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+        tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=USE_TUNE_CACHE)
         lat.collide_and_stream(5, handle)
         ms = lat.time_steps(steps, handle) / steps
         res[st] = {"glups": size ** 3 / (ms * 1e-3) / 1e9, "ms_per_step": ms,
@@ -267,7 +279,7 @@ def time_slab(torch, steps, storage="f32", shape=(64, 512, 512)):
     case.apply(lat)
     stream = torch.cuda.current_stream()
     handle = stream.cuda_stream
-    tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=False)
+    tuned = autotune.tune_lattice(lat, steps=3, stream=handle, use_cache=USE_TUNE_CACHE)
     nbytes = lat.halo_bytes(0)
     lo, hi = lat.halo_buffer(0), lat.halo_buffer(0)
 
@@ -362,7 +374,11 @@ def load_traffic(storage):
 
 
 def main():
+    global USE_TUNE_CACHE
     args = parse_args()
+    if args.tune_cache:
+        os.environ["OLBGFX_TUNE_CACHE"] = os.path.abspath(args.tune_cache)
+        USE_TUNE_CACHE = True
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -392,7 +408,9 @@ def main():
             ens[st] = time_ensemble(torch, st, args.ens_steps, args.warmup, tune=not args.no_tune)
             print(f"# ensemble {st}: graph {ens[st]['graph_steps_per_s']:.1f} steps/s "
                   f"({ens[st]['graph_glups']:.2f} GLUPS), separate {ens[st]['separate_steps_per_s']:.1f} "
-                  f"steps/s, x{ens[st]['speedup_graph_vs_separate']:.2f}", file=sys.stderr)
+                  f"steps/s, x{ens[st]['speedup_graph_vs_separate']:.2f}; separate and synchronised "
+                  f"{ens[st]['separate_synchronized_steps_per_s']:.1f} steps/s, "
+                  f"x{ens[st]['speedup_graph_vs_separate_synchronized']:.2f}", file=sys.stderr)
     porous = None if args.no_porous else time_porous(torch, args.porous_steps)
     if porous:
         print(f"# porous: f32 {porous['f32']['glups']:.2f} GLUPS, f16 {porous['f16']['glups']:.2f} GLUPS",

@@ -22,6 +22,7 @@ extern "C" {
 
 typedef struct olbgfx_lattice olbgfx_lattice;
 typedef struct olbgfx_ensemble olbgfx_ensemble;
+typedef struct olbgfx_cell_list olbgfx_cell_list;
 typedef int32_t olbgfx_status;
 
 #define OLBGFX_OK              0
@@ -243,7 +244,10 @@ olbgfx_status olbgfx_ensemble_cells(const olbgfx_ensemble* ensemble, uint64_t* c
 
 /* ---- timing (HIP events on `stream`; these synchronise) -------------------
  * elapsed milliseconds of `steps` steps; for ensembles mode 0 = persistent
- * kernel per step, 1 = replay of the captured graph, 2 = separate launches. */
+ * kernel per step, 1 = replay of the captured graph, 2 = separate launches,
+ * 3 = separate launches with a synchronisation after each member's step (the
+ * way SuperLattice::collideAndStream waits between its stages,
+ * core/superLattice.hh:663-725). */
 olbgfx_status olbgfx_time_steps(olbgfx_lattice* lattice, uint32_t steps, void* stream, float* ms);
 olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* ensemble, int32_t mode, uint32_t steps,
                                    void* stream, float* ms);
@@ -271,6 +275,31 @@ olbgfx_status olbgfx_halo_pack(olbgfx_lattice* lattice, int32_t axis, int32_t si
                                void* device_buffer, void* stream);
 olbgfx_status olbgfx_halo_unpack(olbgfx_lattice* lattice, int32_t axis, int32_t side,
                                  const void* device_buffer, void* stream);
+
+/* ---- cell lists ----------------------------------------------------------------
+ * ConcreteCommunicatable<CyclicColumn>::size / serialize / deserialize for
+ * all 19 populations (core/platform/gpu/hip/column.h:205-254; the GPU
+ * communicator's gather and scatter of overlap cells,
+ * core/platform/gpu/hip/communicator.hh:529-585): a list of CellIDs (copied
+ * to the device once; ids beyond the padded block are rejected), and the
+ * populations of those cells at the current offsets to or from a device
+ * buffer, population-major (buffer[i * n + k] = f_bar_i(cells[k])), raw
+ * storage words.  Gather and scatter are enqueued on `stream`. */
+olbgfx_status olbgfx_cell_list_create(const olbgfx_lattice* lattice, const uint32_t* cells, uint32_t n,
+                                      olbgfx_cell_list** out);
+olbgfx_status olbgfx_cell_list_destroy(olbgfx_cell_list* list);
+olbgfx_status olbgfx_cell_list_bytes(const olbgfx_lattice* lattice, const olbgfx_cell_list* list,
+                                     uint64_t* bytes);
+olbgfx_status olbgfx_gather_cells(olbgfx_lattice* lattice, const olbgfx_cell_list* list,
+                                  void* device_buffer, void* stream);
+olbgfx_status olbgfx_scatter_cells(olbgfx_lattice* lattice, const olbgfx_cell_list* list,
+                                   const void* device_buffer, void* stream);
+
+/* device memory owned by the library, for halo planes, gathered cells or
+ * RCCL transfers (no reference counterpart: OpenLB allocates these buffers
+ * inside its communicator) */
+olbgfx_status olbgfx_device_buffer_create(uint64_t bytes, void** device_buffer);
+olbgfx_status olbgfx_device_buffer_destroy(void* device_buffer);
 
 /* ---- autotuner ---------------------------------------------------------------
  * Times candidate launch configurations on the lattice (ensemble) itself

@@ -127,6 +127,13 @@ def library() -> ctypes.CDLL:
         "olbgfx_ensemble_time": [vp, i32, u32, vp, ctypes.POINTER(ctypes.c_float)],
         "olbgfx_autotune_lattice": [vp, u32, vp, ctypes.c_char_p, vp, vp],
         "olbgfx_halo_bytes": [vp, i32, ctypes.POINTER(ctypes.c_uint64)],
+        "olbgfx_cell_list_create": [vp, vp, u32, ctypes.POINTER(vp)],
+        "olbgfx_cell_list_destroy": [vp],
+        "olbgfx_cell_list_bytes": [vp, vp, ctypes.POINTER(ctypes.c_uint64)],
+        "olbgfx_gather_cells": [vp, vp, vp, vp],
+        "olbgfx_scatter_cells": [vp, vp, vp, vp],
+        "olbgfx_device_buffer_create": [ctypes.c_uint64, ctypes.POINTER(vp)],
+        "olbgfx_device_buffer_destroy": [vp],
         "olbgfx_halo_buffer_create": [vp, i32, ctypes.POINTER(vp)],
         "olbgfx_halo_buffer_destroy": [vp],
         "olbgfx_halo_pack": [vp, i32, i32, vp, vp],
@@ -333,6 +340,23 @@ class Lattice:
         _check(library().olbgfx_halo_buffer_create(self._h, int(axis), ctypes.byref(p)))
         return HaloBuffer(p.value, self.halo_bytes(axis))
 
+    def cell_list(self, cells) -> "CellList":
+        """CellIDs whose 19 populations gather_cells / scatter_cells move"""
+        ids = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
+        h = ctypes.c_void_p()
+        _check(library().olbgfx_cell_list_create(self._h, _ptr(ids), int(ids.size), ctypes.byref(h)))
+        n = ctypes.c_uint64()
+        _check(library().olbgfx_cell_list_bytes(self._h, h, ctypes.byref(n)))
+        return CellList(h.value, int(ids.size), n.value)
+
+    def gather_cells(self, cells: "CellList", device_ptr: int, stream=None) -> None:
+        _check(library().olbgfx_gather_cells(self._h, ctypes.c_void_p(cells.handle), ctypes.c_void_p(device_ptr),
+                                             stream))
+
+    def scatter_cells(self, cells: "CellList", device_ptr: int, stream=None) -> None:
+        _check(library().olbgfx_scatter_cells(self._h, ctypes.c_void_p(cells.handle), ctypes.c_void_p(device_ptr),
+                                              stream))
+
     def halo_pack(self, axis: int, side: int, device_ptr: int, stream=None) -> None:
         """gather the populations that left through face (axis, side) into the
         device buffer at device_ptr"""
@@ -381,8 +405,35 @@ class Lattice:
         return rho, u.reshape(self.ncells, 3)
 
 
+class CellList:
+    """a device copy of CellIDs (olbgfx_cell_list_create)"""
+
+    def __init__(self, handle: int, n: int, nbytes: int):
+        self.handle = handle
+        self.n = n
+        self.nbytes = nbytes
+
+    def close(self) -> None:
+        if self.handle:
+            library().olbgfx_cell_list_destroy(ctypes.c_void_p(self.handle))
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_buffer(nbytes: int) -> "HaloBuffer":
+    """library-owned device memory (olbgfx_device_buffer_create)"""
+    p = ctypes.c_void_p()
+    _check(library().olbgfx_device_buffer_create(int(nbytes), ctypes.byref(p)))
+    return HaloBuffer(p.value, int(nbytes))
+
+
 class HaloBuffer:
-    """device memory for one packed halo plane (olbgfx_halo_buffer_create)"""
+    """device memory for one packed halo plane or gathered cells"""
 
     def __init__(self, ptr: int, nbytes: int):
         self.ptr = ptr
@@ -453,9 +504,10 @@ class Ensemble:
         _check(library().olbgfx_ensemble_launch_graph(self._h, int(launches), stream))
 
     def time(self, mode: str, steps: int, stream: int | None = None) -> float:
-        """milliseconds of `steps` ensemble steps; mode 'persistent', 'graph'
-        or 'separate' (HIP events on `stream`, synchronises)"""
-        code = {"persistent": 0, "graph": 1, "separate": 2}[mode]
+        """milliseconds of `steps` ensemble steps; mode 'persistent', 'graph',
+        'separate' or 'separate_synchronized' (a synchronisation after each
+        member's step) (HIP events on `stream`, synchronises)"""
+        code = {"persistent": 0, "graph": 1, "separate": 2, "separate_synchronized": 3}[mode]
         ms = ctypes.c_float()
         _check(library().olbgfx_ensemble_time(self._h, code, int(steps), stream, ctypes.byref(ms)))
         return float(ms.value)

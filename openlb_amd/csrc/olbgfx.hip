@@ -46,6 +46,11 @@ constexpr int Q = 19;
 // registers per lane below what the D3Q19 collision needs without spilling
 constexpr uint32_t kMaxBlock = 256;
 constexpr int kMaskedBoundary = -2;  // step-kernel MODE: masked, LocalVelocity kinds compiled in
+
+__host__ __device__ constexpr bool is_velocity_boundary(int kind)
+{
+  return kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_FIXED_VELOCITY;
+}
 // LDS of one gfx950 compute unit
 constexpr uint32_t kLdsPerCu = 160 * 1024;
 // persistent blocks per ensemble launch at most (bounds the ticket atomics)
@@ -747,7 +752,7 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
     counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
   } else if constexpr (MODE == kMaskedBoundary) {
     const DynEntry<A> e = a.table[id];
-    if (kind >= OLBGFX_LOCAL_VELOCITY) {
+    if (is_velocity_boundary(kind)) {
       counted = apply_velocity_boundary(f, e, n, a, st_rho, st_usqr);
     } else {
       counted = apply_any(f, e, n, a, st_rho, st_usqr);
@@ -1574,7 +1579,7 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
   }
   bool any = false;
   for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
-    any = any || L->table[k].kind >= OLBGFX_LOCAL_VELOCITY;
+    any = any || is_velocity_boundary(L->table[k].kind);
   }
   std::vector<uint32_t> edges, corners;
   bool used = false;
@@ -1586,7 +1591,7 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
         const int c[3] = {x + D.padding, y + D.padding, z + D.padding};
         const uint32_t n = ((uint32_t)c[0] * L->py + (uint32_t)c[1]) * L->pz + (uint32_t)c[2];
         const int kind = L->table[L->host_mask[n]].kind;
-        if (kind < OLBGFX_LOCAL_VELOCITY) {
+        if (!is_velocity_boundary(kind)) {
           continue;
         }
         used = true;
@@ -2653,8 +2658,27 @@ olbgfx_status olbgfx_ensemble_time(olbgfx_ensemble* E, int32_t mode, uint32_t st
         s, [&] { return olbgfx_ensemble_launch_graph(E, steps / E->graph_steps, stream); }, ms);
   case 2:
     return time_on_stream(s, [&] { return olbgfx_ensemble_step_separately(E, steps, stream); }, ms);
+  case 3:
+    // each member's step followed by a synchronisation, the way
+    // SuperLattice::collideAndStream waits for the device between stages
+    // (core/superLattice.hh:663-725, gpu/hip/operator.hh:597-624)
+    return time_on_stream(
+        s,
+        [&]() -> olbgfx_status {
+          for (uint32_t k = 0; k < steps; ++k) {
+            for (olbgfx_lattice* L : E->members) {
+              const olbgfx_status st = olbgfx_collide_and_stream(L, 1, stream);
+              if (st != OLBGFX_OK) {
+                return st;
+              }
+              HIP_TRY(hipStreamSynchronize(s));
+            }
+          }
+          return OLBGFX_OK;
+        },
+        ms);
   default:
-    return fail(OLBGFX_ERR_INVALID, "mode: 0 persistent, 1 graph, 2 separate launches");
+    return fail(OLBGFX_ERR_INVALID, "mode: 0 persistent, 1 graph, 2 separate launches, 3 separate and synchronised");
   }
 }
 
@@ -2826,6 +2850,182 @@ olbgfx_status olbgfx_halo_unpack(olbgfx_lattice* L, int32_t axis, int32_t side, 
                                  void* stream)
 {
   return halo(L, axis, side, const_cast<void*>(device_buffer), stream, false);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cell lists: all 19 populations of listed cells to and from a device buffer
+// (the data a communicator moves for an overlap of arbitrary shape)
+
+struct olbgfx_cell_list {
+  uint32_t* cells = nullptr;  // device copy of the CellIDs
+  uint32_t n = 0;
+  uint32_t count = 0;         // padded cells of the lattice the list was made for
+};
+
+namespace {
+
+struct CellCopyArgs {
+  uint32_t shift[Q];
+  uint32_t count;
+  uint32_t n;
+};
+
+template <typename S, bool GATHER>
+__global__ void __launch_bounds__(256) k_cells(void* pop, uint64_t stride, const CellCopyArgs c,
+                                               const uint32_t* cells, S* buf)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)Q * c.n) {
+    return;
+  }
+  const uint32_t i = (uint32_t)(t / c.n), k = (uint32_t)(t - (uint64_t)i * c.n);
+  const uint32_t cell = cells[k];
+  if (cell >= c.count) {
+    return;  // lists are checked when made; a foreign list cannot reach outside the rings
+  }
+  S* at = ring_at<S>(pop, stride, (int)i, ring_slot(cell, c.shift[i], c.count));
+  if constexpr (GATHER) {
+    buf[t] = *at;
+  } else {
+    *at = buf[t];
+  }
+}
+
+template <typename S>
+void cells_launch(olbgfx_lattice* L, const olbgfx_cell_list* list, void* buf, bool gather, hipStream_t s)
+{
+  CellCopyArgs c{};
+  for (int i = 0; i < Q; ++i) c.shift[i] = L->shift[i];
+  c.count = L->count;
+  c.n = list->n;
+  const uint64_t total = (uint64_t)Q * list->n;
+  const dim3 grid((uint32_t)((total + 255) / 256)), block(256);
+  if (gather) {
+    k_cells<S, true><<<grid, block, 0, s>>>(L->pop, L->stride, c, list->cells, (S*)buf);
+  } else {
+    k_cells<S, false><<<grid, block, 0, s>>>(L->pop, L->stride, c, list->cells, (S*)buf);
+  }
+}
+
+olbgfx_status cells_copy(olbgfx_lattice* L, const olbgfx_cell_list* list, void* buf, void* stream, bool gather)
+{
+  if (check(L) != OLBGFX_OK || !list || !buf) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  if (list->count != L->count) {
+    return fail(OLBGFX_ERR_INVALID, "the cell list was made for a lattice of another size");
+  }
+  if (list->n == 0) {
+    return OLBGFX_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  switch (L->desc.storage) {
+  case OLBGFX_STORE_F32:
+    cells_launch<float>(L, list, buf, gather, s);
+    break;
+  case OLBGFX_STORE_F64:
+    cells_launch<double>(L, list, buf, gather, s);
+    break;
+  default:
+    cells_launch<uint16_t>(L, list, buf, gather, s);
+    break;
+  }
+  HIP_TRY(hipGetLastError());
+  return OLBGFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_cell_list_create(const olbgfx_lattice* L, const uint32_t* cells, uint32_t n,
+                                      olbgfx_cell_list** out)
+{
+  if (!L || !out || (n > 0 && !cells)) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  *out = nullptr;
+  for (uint32_t k = 0; k < n; ++k) {
+    if (cells[k] >= L->count) {
+      return fail(OLBGFX_ERR_INVALID, "cell id outside the padded block");
+    }
+  }
+  olbgfx_cell_list* list = new (std::nothrow) olbgfx_cell_list();
+  if (!list) {
+    return fail(OLBGFX_ERR_NOMEM, "out of host memory");
+  }
+  list->n = n;
+  list->count = L->count;
+  if (n > 0) {
+    if (hipMalloc(&list->cells, sizeof(uint32_t) * n) != hipSuccess) {
+      (void)hipGetLastError();
+      delete list;
+      return fail(OLBGFX_ERR_NOMEM, "cell list allocation failed");
+    }
+    if (hipMemcpy(list->cells, cells, sizeof(uint32_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(list->cells);
+      delete list;
+      return fail(OLBGFX_ERR_HIP, "cell list upload failed");
+    }
+  }
+  *out = list;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_cell_list_destroy(olbgfx_cell_list* list)
+{
+  if (list) {
+    if (list->cells) {
+      HIP_TRY(hipFree(list->cells));
+    }
+    delete list;
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_cell_list_bytes(const olbgfx_lattice* L, const olbgfx_cell_list* list, uint64_t* bytes)
+{
+  if (!L || !list || !bytes) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  *bytes = (uint64_t)Q * list->n * (uint64_t)L->elem;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_gather_cells(olbgfx_lattice* L, const olbgfx_cell_list* list, void* device_buffer,
+                                  void* stream)
+{
+  return cells_copy(L, list, device_buffer, stream, true);
+}
+
+olbgfx_status olbgfx_scatter_cells(olbgfx_lattice* L, const olbgfx_cell_list* list, const void* device_buffer,
+                                   void* stream)
+{
+  return cells_copy(L, list, const_cast<void*>(device_buffer), stream, false);
+}
+
+olbgfx_status olbgfx_device_buffer_create(uint64_t bytes, void** device_buffer)
+{
+  if (!device_buffer) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  *device_buffer = nullptr;
+  if (hipMalloc(device_buffer, std::max<uint64_t>(bytes, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    *device_buffer = nullptr;
+    return fail(OLBGFX_ERR_NOMEM, "device buffer allocation failed");
+  }
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_device_buffer_destroy(void* device_buffer)
+{
+  if (device_buffer) {
+    HIP_TRY(hipFree(device_buffer));
+  }
+  return OLBGFX_OK;
 }
 
 }  // extern "C"

@@ -8,7 +8,7 @@
  *
  *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
  *            [--STORAGE f32|f16|bf16] [--DUMP file] [--TUNE_CACHE file]
- *            [--BOUNDARY local_velocity|bounce_back]
+ *            [--BOUNDARY local_velocity|bounce_back] [--SAVE file] [--LOAD file]
  *
  * --DUMP writes the materials and the populations of the padded block after
  * the run (logical order, float32) for the tests.
@@ -35,6 +35,8 @@ struct Parameters {
   std::string dump;
   std::string tuneCache;  // "" = no autotuning
   bool localVelocity = true;  // cavity3d.cpp:111-124: LID_DRIVEN without LID_DRIVEN_BOUNCE_BACK
+  std::string save;           // checkpoint after the timed steps
+  std::string load;           // checkpoint to restart from before the warm-up
 };
 
 static Parameters fromCLI(int argc, char** argv)
@@ -48,6 +50,8 @@ static Parameters fromCLI(int argc, char** argv)
     else if (k == "--REYNOLDS") p.reynolds = std::stof(v);
     else if (k == "--DUMP") p.dump = v;
     else if (k == "--TUNE_CACHE") p.tuneCache = v;
+    else if (k == "--SAVE") p.save = v;
+    else if (k == "--LOAD") p.load = v;
     else if (k == "--BOUNDARY") {
       if (v != "local_velocity" && v != "bounce_back") {
         throw std::invalid_argument("--BOUNDARY is local_velocity or bounce_back");
@@ -107,6 +111,9 @@ int main(int argc, char** argv)
     SuperLattice<T, DESCRIPTOR> lattice(geometry, p.storage);
     prepareLattice(lattice, geometry, p);
     setInitialValues(lattice, geometry, p);
+    if (!p.load.empty()) {
+      lattice.load(p.load);
+    }
     if (!p.tuneCache.empty()) {
       const olbgfx_tune_trial t = lattice.autotune(p.tuneCache);
       std::cerr << "autotune: block " << t.block_threads << ", waves/SIMD " << t.waves_per_simd
@@ -125,6 +132,9 @@ int main(int argc, char** argv)
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const double mlups = double(geometry.getNvoxel()) * p.timeSteps / s / 1e6;
 
+    if (!p.save.empty()) {
+      lattice.save(p.save);
+    }
     lattice.setProcessingContext(ProcessingContext::Evaluation);
     std::cout << p.resolution << ", " << p.timeSteps << ", 1, 1, 1, " << mlups << std::endl;
     if (!p.dump.empty()) {

@@ -12,6 +12,7 @@
  *   SuperLattice::initialize, collideAndStream core/superLattice.hh:248-266, 663-725
  *   setStatisticsOn/Off, getStatistics         core/superLattice.h, core/latticeStatistics.h:43-127
  *   setProcessingContext(Evaluation)           core/superLattice.hh:184-195
+ *   save / load (OpenLB checkpoint format)     core/serializer.h:181-189, io/serializerIO.hh:40-64
  *   autotune (an addition of this platform)    no counterpart; OpenLB fixes launch shapes
  *
  * Everything below calls the C ABI of include/olbgfx.h; no HIP header is
@@ -26,10 +27,14 @@
 #ifndef OLB_GFX950_H
 #define OLB_GFX950_H
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <fstream>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -240,6 +245,50 @@ struct LatticeStatistics {
   T getAverageEnergy() const { return averageEnergy; }
   T getMaxU() const { return maxU; }
 };
+
+// ---- base64 (RFC 4648 alphabet with padding, as OpenLB's Base64Encoder) ----
+inline std::string base64(const std::uint8_t* data, std::size_t n)
+{
+  static const char* A = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::string out;
+  out.reserve((n + 2) / 3 * 4);
+  for (std::size_t i = 0; i < n; i += 3) {
+    const std::uint32_t v = (std::uint32_t)data[i] << 16 | (i + 1 < n ? (std::uint32_t)data[i + 1] << 8 : 0) |
+                            (i + 2 < n ? (std::uint32_t)data[i + 2] : 0);
+    out += A[(v >> 18) & 63];
+    out += A[(v >> 12) & 63];
+    out += i + 1 < n ? A[(v >> 6) & 63] : '=';
+    out += i + 2 < n ? A[v & 63] : '=';
+  }
+  return out;
+}
+
+inline std::vector<std::uint8_t> unbase64(const std::string& text)
+{
+  auto val = [](char c) -> int {
+    if (c >= 'A' && c <= 'Z') return c - 'A';
+    if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+    if (c >= '0' && c <= '9') return c - '0' + 52;
+    if (c == '+') return 62;
+    if (c == '/') return 63;
+    return -1;
+  };
+  std::vector<std::uint8_t> out;
+  out.reserve(text.size() / 4 * 3);
+  std::uint32_t acc = 0;
+  int bits = 0;
+  for (char c : text) {
+    const int v = val(c);
+    if (v < 0) continue;  // padding and line breaks
+    acc = acc << 6 | (std::uint32_t)v;
+    bits += 6;
+    if (bits >= 8) {
+      bits -= 8;
+      out.push_back((std::uint8_t)(acc >> bits));
+    }
+  }
+  return out;
+}
 
 // ---- discrete normals (geometry/discreteNormals.h:33-57) ------------------
 enum class NormalType { None, Flat, ExternalEdge, ExternalCorner };
@@ -531,6 +580,76 @@ public:
     } else if (_hostValid) {
       check(olbgfx_set_populations(handle(), valueType(), _host.data()));
     }
+  }
+
+  /// Checkpoint in OpenLB's format (SuperLattice::save, core/serializer.h:181,
+  /// io/serializerIO.hh:40-64): base64 of the 8-byte binary size, then
+  /// base64 of the 19 population column blocks [shift][count][values]
+  /// (core/platform/cpu/simd/column.hh:65-87 for CpuSimd, two copies of a
+  /// page-aligned ring; cpu/sisd/column.hh:67-89 for CpuSisd, one ring of
+  /// the padded cell count).  Unlike OpenLB, `fileName` is the whole path.
+  enum class CheckpointLayout { CpuSimd, CpuSisd };
+
+  void save(const std::string& fileName, CheckpointLayout layout = CheckpointLayout::CpuSimd)
+  {
+    setProcessingContext(ProcessingContext::Evaluation);
+    const std::size_t n = _ncells;
+    const std::size_t count = layout == CheckpointLayout::CpuSimd ? ((n * sizeof(T) - 1) / 4096 + 1) * 4096 / sizeof(T) : n;
+    const std::size_t copies = layout == CheckpointLayout::CpuSimd ? 2 : 1;
+    std::vector<std::uint8_t> body;
+    body.reserve(DESCRIPTOR::q * (16 + copies * count * sizeof(T)));
+    auto put = [&](const void* p, std::size_t bytes) {
+      const auto* b = static_cast<const std::uint8_t*>(p);
+      body.insert(body.end(), b, b + bytes);
+    };
+    std::vector<T> ring(count);
+    for (int i = 0; i < DESCRIPTOR::q; ++i) {
+      const std::int64_t shift = 0;
+      const std::uint64_t c = count;
+      put(&shift, 8);
+      put(&c, 8);
+      std::fill(ring.begin(), ring.end(), T(0));
+      std::copy(_host.begin() + (std::ptrdiff_t)(i * n), _host.begin() + (std::ptrdiff_t)((i + 1) * n), ring.begin());
+      for (std::size_t k = 0; k < copies; ++k) put(ring.data(), count * sizeof(T));
+    }
+    const std::uint64_t size = body.size();
+    std::ofstream out(fileName, std::ios::binary);
+    if (!out) throw std::runtime_error("cannot write " + fileName);
+    out << base64(reinterpret_cast<const std::uint8_t*>(&size), 8) << base64(body.data(), body.size());
+  }
+
+  /// restart from a checkpoint of either layout written by SuperLattice::save
+  /// of a lattice of this block's size and value type
+  void load(const std::string& fileName)
+  {
+    std::ifstream in(fileName, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot read " + fileName);
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    if (text.size() < 12) throw std::invalid_argument("not an OpenLB checkpoint");
+    const std::vector<std::uint8_t> head = unbase64(text.substr(0, 12));
+    std::uint64_t size = 0;
+    std::memcpy(&size, head.data(), 8);
+    const std::vector<std::uint8_t> body = unbase64(text.substr(12));
+    if (body.size() != size || size % DESCRIPTOR::q != 0) throw std::invalid_argument("checkpoint size mismatch");
+    const std::size_t block = size / DESCRIPTOR::q, n = _ncells;
+    _host.resize((std::size_t)DESCRIPTOR::q * n);
+    for (int i = 0; i < DESCRIPTOR::q; ++i) {
+      const std::uint8_t* b = body.data() + i * block;
+      std::int64_t shift = 0;
+      std::uint64_t count = 0;
+      std::memcpy(&shift, b, 8);
+      std::memcpy(&count, b + 8, 8);
+      if (count < n || (16 + count * sizeof(T) != block && 16 + 2 * count * sizeof(T) != block)) {
+        throw std::invalid_argument("checkpoint column does not fit this lattice");
+      }
+      const T* data = reinterpret_cast<const T*>(b + 16);
+      for (std::size_t c = 0; c < n; ++c) {
+        const std::int64_t j = ((std::int64_t)c + shift) % (std::int64_t)count;
+        std::memcpy(&_host[(std::size_t)i * n + c], data + (j < 0 ? j + (std::int64_t)count : j), sizeof(T));
+      }
+    }
+    _hostValid = true;
+    setProcessingContext(ProcessingContext::Simulation);
   }
 
   /// population iPop of cell iCell, shifted by -t_i (evaluation mirror)

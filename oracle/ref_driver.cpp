@@ -20,6 +20,10 @@
  *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
  *                                                convergence (the reference's KAT)
  *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
+ *   olb_ref checkpoint N STEPS OUT NAME           TGV after STEPS steps: populations dump and
+ *                                                OpenLB's own checkpoint (SuperLattice::save)
+ *   olb_ref restore N OUT NAME                    TGV lattice loaded from checkpoint NAME
+ *                                                (SuperLattice::load), populations dumped
  *
  * Dump format (little endian): text header line "OLBDUMP v1 <key=value ...>\n",
  * then named arrays, each "ARRAY <name> <dtype> <count>\n" followed by raw data.
@@ -244,7 +248,8 @@ void build_periodic_box(int nx, int ny, int nz, CuboidDecomposition3D<T>*& cuboi
   G->communicate();
 }
 
-int run_tgv(int n, int steps, double u0, double tau, const std::string& out, bool bench, double smagorinsky = -1)
+int run_tgv(int n, int steps, double u0, double tau, const std::string& out, bool bench, double smagorinsky = -1,
+            const std::string& checkpoint = "")
 {
   using DESCRIPTOR = descriptors::D3Q19<>;
   CuboidDecomposition3D<T>* cuboids;
@@ -275,6 +280,27 @@ int run_tgv(int n, int steps, double u0, double tau, const std::string& out, boo
     }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("MLUPS %.3f\nseconds %.3f\n", double(n) * n * n * steps / s / 1e6, s);
+    return 0;
+  }
+  if (!checkpoint.empty() && steps < 0) {
+    // SuperLattice::load: a checkpoint written elsewhere, then the dump of
+    // what the lattice holds
+    L.load(checkpoint);
+    Dump d(out, block_header(L) + " case=tgv_restore n=" + std::to_string(n));
+    d.array("f1", tname(), populations(L));
+    return 0;
+  }
+  if (!checkpoint.empty()) {
+    // SuperLattice::save (core/serializer.h:181): OpenLB's own checkpoint of
+    // the lattice after the steps, next to the dump of its populations
+    for (int t = 0; t < steps; ++t) {
+      L.collideAndStream();
+    }
+    L.save(checkpoint);
+    Dump d(out, block_header(L) + " case=tgv_checkpoint n=" + std::to_string(n) + " steps=" + std::to_string(steps));
+    d.array("materials", "i4", materials(*G, 3));
+    d.array("distances", "i8", distances(L));
+    d.array("f1", tname(), populations(L));
     return 0;
   }
   Dump d(out, block_header(L) + " case=tgv n=" + std::to_string(n) + " steps=" + std::to_string(steps) +
@@ -473,6 +499,12 @@ int main(int argc, char** argv)
     if (mode == "tgv_smagorinsky" && argc == 8) {
       return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[7], false,
                      std::stod(argv[6]));
+    }
+    if (mode == "restore" && argc == 5) {
+      return run_tgv(std::stoi(argv[2]), -1, 0.04, 0.56, argv[3], false, -1, argv[4]);
+    }
+    if (mode == "checkpoint" && argc == 6) {
+      return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), 0.04, 0.56, argv[4], false, -1, argv[5]);
     }
     if (mode == "bench_tgv" && argc == 4) {
       return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), 0.04, 0.56, "", true);

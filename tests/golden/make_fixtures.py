@@ -4,7 +4,7 @@ path (oracle/_ref, built by `make -C oracle ref` from the OpenLB sources).
 Run in the build container (the OpenLB sources are not on the GPU host):
     python tests/golden/make_fixtures.py            # the small fixtures
     python tests/golden/make_fixtures.py NAME ...   # full-size ones (FULL), or
-                                                    # local_velocity, smagorinsky
+                                                    # local_velocity, smagorinsky, checkpoint
 Each fixture holds the materials of the padded block (CellID order), the
 neighbour distances, the populations at t = 0 and after the run for the core
 cells (logical order, shifted by -t_i), and the run's parameters.
@@ -87,6 +87,41 @@ def smagorinsky_fixtures() -> None:
         path = os.path.join(RUN, f"tgv_smagorinsky_{tag}.bin")
         run(binary, ["tgv_smagorinsky", "12", "100", "0.04", "0.505", "0.1", path])
         save(f"tgv12_smagorinsky_{tag}", *read_dump(path))
+
+
+def checkpoint_fixtures() -> None:
+    """OpenLB's checkpoint format (SuperLattice::save / load, CPU_SIMD
+    columns, FP32) on a periodic TGV 4^3 (10^3 padded cells):
+      - the reference saves after 3 steps: tgv4_openlb_checkpoint_f32.dat,
+        with its populations in tgv4_checkpoint_f32.npz ("f1");
+      - the reference loads a checkpoint that openlb_amd.checkpoint.write
+        made from seeded populations: the npz keeps the seed, the SHA-256 of
+        that file and what the reference read back ("restored")."""
+    import hashlib
+    import shutil
+
+    from openlb_amd import checkpoint
+
+    tmp = os.path.join(RUN, "tmp")
+    os.makedirs(tmp, exist_ok=True)
+    dump = os.path.join(RUN, "ck_ref.bin")
+    run("olb_ref_f", ["checkpoint", "4", "3", dump, "ck_ref"])
+    shutil.copyfile(os.path.join(tmp, "ck_ref_rank0_size1.dat"), os.path.join(HERE, "tgv4_openlb_checkpoint_f32.dat"))
+    meta, arrays = read_dump(dump)
+    n = np.asarray(arrays["materials"]).size
+    seed = 20261202
+    f = (np.random.default_rng(seed).standard_normal((19, n)) * 1e-2).astype(np.float32)
+    ours = os.path.join(tmp, "ck_ours_rank0_size1.dat")
+    checkpoint.write(ours, f, "cpu_simd", np.float32)
+    digest = hashlib.sha256(open(ours, "rb").read()).hexdigest()
+    back = os.path.join(RUN, "ck_back.bin")
+    run("olb_ref_f", ["restore", "4", back, "ck_ours"])
+    _, restored = read_dump(back)
+    np.savez_compressed(os.path.join(HERE, "tgv4_checkpoint_f32.npz"),
+                        meta=np.array(json.dumps(dict(meta, seed=seed, sha256=digest))),
+                        f1=np.asarray(arrays["f1"]).reshape(19, n),
+                        restored=np.asarray(restored["f1"]).reshape(19, n))
+    print("wrote checkpoint fixtures", meta)
 
 
 def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode: str = "cavity") -> None:
@@ -189,11 +224,14 @@ def main() -> None:
                 cavity_reduced("olb_ref_f", "f32", mode="cavity_lv")
             elif name == "smagorinsky":
                 smagorinsky_fixtures()
+            elif name == "checkpoint":
+                checkpoint_fixtures()
             else:
                 full_size(name)
         return
     local_velocity_fixtures()
     smagorinsky_fixtures()
+    checkpoint_fixtures()
     cavity_reduced("olb_ref_f", "f32", mode="cavity_lv")
     cavity_reduced("olb_ref_f", "f32")
     for tag, binary in (("f32", "olb_ref_f"), ("f64", "olb_ref_d")):

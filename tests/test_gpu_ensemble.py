@@ -182,3 +182,18 @@ def test_ensemble_block_orders_give_identical_results(order, tiles):
     eb.step(9)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+
+
+@pytest.mark.parametrize("mode", ["persistent", "separate", "separate_synchronized"])
+def test_timed_modes_advance_like_members_alone(mode):
+    """each timing mode runs real steps: afterwards the members equal lattices
+    stepped alone as often"""
+    _, a = _members("f32")
+    _, b = _members("f32")
+    ens = Ensemble(a)
+    ms = ens.time(mode, 6)
+    assert ms > 0
+    for lat in b:
+        lat.collide_and_stream(6)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations())

@@ -45,3 +45,29 @@ def test_cpp_cavity_matches_oracle(tmp_path, boundary):
 def test_cpp_rejects_bad_options():
     out = subprocess.run([BINARY, "--BOGUS", "1"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "unknown option" in out.stderr
+
+
+def test_cpp_checkpoint_save_and_restart(tmp_path):
+    """the mirror's SuperLattice::save writes OpenLB's checkpoint format (the
+    Python reader, pinned against the reference, reads the dumped
+    populations back), and restarting from it continues the run exactly"""
+    from openlb_amd import checkpoint
+
+    res = 8
+    c = cases.cavity3d_local_velocity(res, dtype=np.float32)
+
+    def run(steps, *extra):
+        dump = tmp_path / f"d{steps}_{len(extra)}.bin"
+        out = subprocess.run([BINARY, "--RESOLUTION", str(res), "--TIME_STEPS", str(steps), "--DUMP", str(dump),
+                              *extra], capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        raw = np.fromfile(dump, dtype=np.uint8)
+        return raw[c.ncells:].view(np.float32).reshape(19, c.ncells)
+
+    ck = tmp_path / "ck.dat"
+    f5 = run(5, "--SAVE", str(ck))
+    np.testing.assert_array_equal(checkpoint.read(str(ck), c.ncells), f5)
+    f_restart = run(5, "--LOAD", str(ck))      # 10 warm-up + 5 more after the restart
+    f_straight = run(20)                       # 10 warm-up + 20
+    core = c.materials.reshape(-1) != 0
+    np.testing.assert_array_equal(f_restart[:, core], f_straight[:, core])

@@ -1,6 +1,7 @@
 """Host-side logic of the C++ mirror that needs no GPU: its discrete-normal
 classification of the cavity walls equals the reference's (the golden
-fixture holds the normals OpenLB's boundary::set computed)."""
+fixture holds the normals OpenLB's boundary::set computed), and its base64
+codec reproduces an OpenLB checkpoint."""
 import os
 import subprocess
 
@@ -27,3 +28,16 @@ def test_cpp_discrete_normals_match_reference(tmp_path):
     for x, y, z, t, nx, ny, nz in rows:
         assert types[x + p, y + p, z + p] == t
         assert tuple(normals[x + p, y + p, z + p]) == (nx, ny, nz)
+
+
+def test_cpp_base64_reproduces_reference_checkpoint(tmp_path):
+    """the mirror's base64 decoder reads OpenLB's checkpoint text and its
+    encoder writes the same text back, byte for byte"""
+    exe = tmp_path / "base64_check"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "openlb_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "base64_check.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), os.path.join(ROOT, "tests", "golden", "tgv4_openlb_checkpoint_f32.dat")],
+                         capture_output=True, text=True, check=True).stdout.split()
+    assert out[0] == out[1] == str(19 * (16 + 2 * 1024 * 4))
+    assert out[2] == "same"
