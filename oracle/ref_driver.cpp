@@ -19,6 +19,7 @@
  *                                                periodic sphere pack, ForcedTRT + BounceBack
  *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
  *                                                convergence (the reference's KAT)
+ *   olb_ref poiseuille_bouzidi N OUT              the same with the Bouzidi wall (its bouzidi row)
  *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
  *   olb_ref checkpoint N STEPS OUT NAME           TGV after STEPS steps: populations dump and
  *                                                OpenLB's own checkpoint (SuperLattice::save)
@@ -375,7 +376,7 @@ int run_porous(int nx, int ny, int nz, const std::string& spheres, double radius
 // ---- Poiseuille pipe, forced BGK + BounceBack, run to convergence: the set-up
 // of the reference's known-answer test tests/benchmarks/testPoiseuille3DBGK.cpp
 // (forced, bounceBack row), driven through the same public API ----
-int run_poiseuille(int n, const std::string& out)
+int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
   const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
@@ -400,7 +401,17 @@ int run_poiseuille(int n, const std::string& out)
   SuperLattice<T,DESCRIPTOR> L(converter, G);
   L.template defineDynamics<NoDynamics>(G, 0);
   L.template defineDynamics<ForcedBGKdynamics>(G, 1);
-  L.template defineDynamics<BounceBack>(G, 2);
+  if (bouzidi) {
+    // the bouzidi row of the test (testPoiseuille3DBGK.cpp:141-151, forced)
+    L.template defineDynamics<NoDynamics>(G, 2);
+    Vector<T,3> b0(0, r, r), b1(lx, r, r);
+    b0[0] -= 0.5 * dx + 3. * dx;
+    b1[0] += 0.5 * dx + 3. * dx;
+    IndicatorCylinder3D<T> wall(b0, b1, r);
+    setBouzidiBoundary<T, DESCRIPTOR, BouzidiPostProcessor>(L, G, 2, wall);
+  } else {
+    L.template defineDynamics<BounceBack>(G, 2);
+  }
   const T D = converter.getLatticeLength(diameter);
   std::vector<T> f(3, T());
   f[0] = 4. * converter.getLatticeViscosity() * converter.getCharLatticeVelocity() / (D * D / 4.);
@@ -463,9 +474,10 @@ int run_poiseuille(int n, const std::string& out)
   }
   char buf[512];
   std::snprintf(buf, sizeof(buf),
-                " case=poiseuille n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g deltaT=%zu "
+                " case=poiseuille boundary=%s n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g deltaT=%zu "
                 "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu",
-                n, iT, (double)omega, (double)f[0], (double)converter.getConversionFactorVelocity(),
+                bouzidi ? "bouzidi" : "bounceBack", n, iT, (double)omega, (double)f[0],
+                (double)converter.getConversionFactorVelocity(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2,
                 converter.getLatticeTime(maxPhysT));
   Dump d(out, block_header(L) + buf);
@@ -475,6 +487,17 @@ int run_poiseuille(int n, const std::string& out)
   d.array("f1", tname(), populations(L));
   d.array("energy", "f8", energy);
   d.array("usol", tname(), usol);
+  if (bouzidi) {
+    // the Bouzidi distances of every cell, population-major (-1: none)
+    auto& b = L.getBlock(0);
+    const std::size_t nc = b.getNcells();
+    std::vector<T> q(DESCRIPTOR::q * nc);
+    for (std::size_t c = 0; c < nc; ++c) {
+      const auto v = b.get(c).template getField<descriptors::BOUZIDI_DISTANCE>();
+      for (int i = 0; i < DESCRIPTOR::q; ++i) q[i * nc + c] = v[i];
+    }
+    d.array("bouzidi", tname(), q);
+  }
   return 0;
 }
 
@@ -511,6 +534,9 @@ int main(int argc, char** argv)
     }
     if (mode == "poiseuille" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3]);
+    }
+    if (mode == "poiseuille_bouzidi" && argc == 4) {
+      return run_poiseuille(std::stoi(argv[2]), argv[3], true);
     }
     if (mode == "porous" && argc == 12) {
       return run_porous(std::stoi(argv[2]), std::stoi(argv[3]), std::stoi(argv[4]), argv[5], std::stod(argv[6]),

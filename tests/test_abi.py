@@ -50,3 +50,14 @@ def test_error_reporting_without_device():
     desc.nx = desc.ny = desc.nz = 1300
     assert lib.olbgfx_lattice_create(ctypes.byref(desc), ctypes.byref(h)) == 1
     assert b"2^31" in lib.olbgfx_last_error()
+    desc.nx = desc.ny = desc.nz = 1100          # 1.34e9 cells: below 2^31, but 5.3 GB per FP32 ring
+    assert lib.olbgfx_lattice_create(ctypes.byref(desc), ctypes.byref(h)) == 1
+    assert b"4 GiB" in lib.olbgfx_last_error()
+    desc.storage, desc.nx, desc.ny, desc.nz = 1, 1100, 1100, 1100   # FP16 rings of 2.7 GB fit
+    desc.nx = 0
+    assert lib.olbgfx_lattice_create(ctypes.byref(desc), ctypes.byref(h)) == 1
+    assert b"positive" in lib.olbgfx_last_error()
+    out = ctypes.c_void_p()
+    assert lib.olbgfx_cell_list_create(None, None, 0, ctypes.byref(out)) == 1
+    assert lib.olbgfx_device_buffer_create(16, None) == 1
+    assert lib.olbgfx_autotune_lattice(None, 4, None, None, None, None) == 1
