@@ -140,7 +140,23 @@ olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* lattice, int32_t field,
  * fails with OLBGFX_ERR_INVALID if a LOCAL_VELOCITY cell has no flat normal. */
 olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* lattice, const int8_t* normals);
 
-/* ---- populations -----------------------------------------------------------
+/* setBouzidiBoundary<T, DESCRIPTOR, BouzidiPostProcessor>
+ * (boundary/setBouzidiBoundary.h:193-330): the cells that run
+ * BouzidiPostProcessor (setBouzidiBoundary.h:48-80) after each stream, before
+ * the LocalVelocity processors (its priority -1), and their BOUZIDI_DISTANCE
+ * field (boundary/bouzidiFields.h:32-36).  cells: n core cell ids in
+ * ascending order; q: 19 distances per cell (q[19*k + i] for direction i of
+ * cells[k]; < 0 none, 0 full-way bounce-back, > 0 interpolated) in value_type.
+ * The reference applies the cells one after another; here they are updated
+ * concurrently, so a list in which one cell's update reads a population that
+ * another's writes is rejected (OLBGFX_ERR_INVALID) -- the setter's q = 0.5
+ * pairs between two solid cells resolve in ascending-id order before the
+ * call.  n = 0 removes them.  Lattices with Bouzidi cells cannot join an
+ * ensemble (OLBGFX_ERR_UNSUPPORTED). */
+olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* lattice, int64_t n, const int64_t* cells,
+                                 int32_t value_type, const void* q);
+
+/* ---- populations -----------------------------------------------------------/* ---- populations -----------------------------------------------------------
  * iniEquilibrium(indicator, rho, u) (core/blockLattice.hh:202-236,
  * dynamics/interface.h:111-121): f_bar_i = feq_i(rho, u) on selected cells.
  * sel: one byte per cell (NULL = all), rho: ncells, u: 3*ncells. */
@@ -173,8 +189,12 @@ olbgfx_status olbgfx_get_statistics(olbgfx_lattice* lattice, double out[4]);
  * core/platform/gpu/hip/statistics.hh:79-115.  Synchronises. */
 olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* lattice, double out[8], void* stream);
 
-/* density and velocity of every padded cell (Cell::computeRhoU), FP32 or
- * FP64 host arrays: rho[ncells], u[3*ncells].  Synchronises. */
+/* density and velocity of every padded cell through the momenta of its
+ * dynamics (Cell::computeRhoU, dynamics/momenta/elements.h): NoDynamics
+ * 1 and 0, BounceBack its RHO (wall_rho) and 0, BounceBackVelocity its RHO
+ * and wall velocity, LocalVelocity the boundary density and wall velocity,
+ * forced dynamics u + F/2, the rest bulk moments.  FP32 or FP64 host arrays:
+ * rho[ncells], u[3*ncells].  Synchronises. */
 olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* lattice, int32_t value_type, void* rho, void* u);
 
 /* ---- launch configuration ---------------------------------------------------

@@ -102,6 +102,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_set_dynamics_mask": [vp, vp],
         "olbgfx_set_cell_field": [vp, i32, i32, vp],
         "olbgfx_set_cell_normals": [vp, vp],
+        "olbgfx_set_bouzidi": [vp, ctypes.c_int64, vp, i32, vp],
         "olbgfx_ini_equilibrium": [vp, vp, i32, vp, vp],
         "olbgfx_get_populations": [vp, i32, vp],
         "olbgfx_set_populations": [vp, i32, vp],
@@ -281,6 +282,17 @@ class Lattice:
         if v.size != 3 * self.ncells:
             raise ValueError("normals must hold 3 entries per cell")
         _check(library().olbgfx_set_cell_normals(self._h, _ptr(v)))
+
+    def set_bouzidi(self, cells, q) -> None:
+        """setBouzidiBoundary: the core cells (ascending ids) that run
+        BouzidiPostProcessor after each stream and their BOUZIDI_DISTANCE
+        (n x 19, < 0 where unset); openlb_amd.bouzidi computes both from a
+        geometry and an indicator.  An empty list removes them."""
+        cells = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if q.size != 19 * cells.size:
+            raise ValueError("q must hold 19 distances per cell")
+        _check(library().olbgfx_set_bouzidi(self._h, int(cells.size), _ptr(cells), VALUE_F64, _ptr(q)))
 
     def ini_equilibrium(self, rho: np.ndarray, u: np.ndarray, sel: np.ndarray | None = None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)

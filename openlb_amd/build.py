@@ -61,7 +61,7 @@ def build_host_examples() -> None:
     out = os.path.join(PKG, "host", "examples", "cavity3d")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
           "-I", os.path.join(PKG, "host"), src, "-L", PKG, "-lolbgfx",
-          "-Wl,-rpath,$ORIGIN/../..", "-o", out])
+          "-Wl,-rpath,$ORIGIN/../..", "-lz", "-o", out])
 
 
 def build_oracle(with_reference: bool = True) -> None:

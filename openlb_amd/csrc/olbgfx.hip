@@ -727,7 +727,9 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
 
   // In the masked path the populations are loaded before the dynamics id is
   // known, so their loads overlap the mask and table loads instead of
-  // waiting behind them; a NoDynamics core cell then stores nothing.
+  // waiting behind them; a NoDynamics core cell then stores nothing but,
+  // on a periodic plane, still hands its crossing populations to the ghost
+  // layer, as the reference's overlap copy does for every cell.
   int kind;
   int id;
   A f[Q];
@@ -744,27 +746,25 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
   if constexpr (MODE < 0) {
     kind = a.table[id].kind;
   }
-  if (kind == OLBGFX_NO_DYNAMICS) {
-    return;
-  }
-
-  if constexpr (MODE >= 0) {
-    counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
-  } else if constexpr (MODE == kMaskedBoundary) {
-    const DynEntry<A> e = a.table[id];
-    if (is_velocity_boundary(kind)) {
-      counted = apply_velocity_boundary(f, e, n, a, st_rho, st_usqr);
+  if (kind != OLBGFX_NO_DYNAMICS) {
+    if constexpr (MODE >= 0) {
+      counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
+    } else if constexpr (MODE == kMaskedBoundary) {
+      const DynEntry<A> e = a.table[id];
+      if (is_velocity_boundary(kind)) {
+        counted = apply_velocity_boundary(f, e, n, a, st_rho, st_usqr);
+      } else {
+        counted = apply_any(f, e, n, a, st_rho, st_usqr);
+      }
     } else {
+      const DynEntry<A> e = a.table[id];
       counted = apply_any(f, e, n, a, st_rho, st_usqr);
     }
-  } else {
-    const DynEntry<A> e = a.table[id];
-    counted = apply_any(f, e, n, a, st_rho, st_usqr);
-  }
 
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
+    for (int i = 0; i < Q; ++i) {
+      *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
+    }
   }
 
   if (a.periodic) {
@@ -913,6 +913,99 @@ __global__ void __launch_bounds__(256) k_velocity_post(const StepArgs<typename S
   for (int i = 0; i < Q; ++i) {
     *ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)) =
         Storage<S>::pack(feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell::computeRhoU of every padded cell through the momenta of its dynamics
+// (dynamics/momenta/elements.h): NoDynamics OneDensity and ZeroMomentum
+// (dynamics.h:52-62); BounceBack FixedDensity (the cell's RHO, 1 unless
+// defined) and ZeroMomentum (dynamics.h:557-568); BounceBackVelocity
+// FixedDensity and FixedVelocityMomentum; LocalVelocity faces the
+// velocity-BM density and the wall velocity, edges and corners the bulk
+// density and the wall velocity; forced dynamics add F/2; the rest are bulk.
+template <typename S, typename H>
+__global__ void __launch_bounds__(256) k_cell_rho_u(const StepArgs<typename Storage<S>::A> a, H* rho_out,
+                                                    H* u_out)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.count) {
+    return;
+  }
+  const DynEntry<A>& e = a.table[a.mask[n]];
+  A rho, u[3];
+  if (e.kind == OLBGFX_NO_DYNAMICS) {
+    rho = A(1);
+    u[0] = u[1] = u[2] = A(0);
+  } else if (e.kind == OLBGFX_BOUNCE_BACK || e.kind == OLBGFX_BOUNCE_BACK_VELOCITY) {
+    rho = a.cell_rho ? a.cell_rho[n] : e.wall_rho;
+    for (int ax = 0; ax < 3; ++ax) {
+      u[ax] = e.kind == OLBGFX_BOUNCE_BACK ? A(0) : wall_component(a, e, n, ax);
+    }
+  } else {
+    rho = momenta_rho<S>(a, n);
+    momenta_u<S>(a, n, u);
+  }
+  rho_out[n] = (H)rho;
+  for (int ax = 0; ax < 3; ++ax) u_out[3 * (size_t)n + ax] = (H)u[ax];
+}
+
+// ---------------------------------------------------------------------------
+// BouzidiPostProcessor (boundary/setBouzidiBoundary.h:48-80), PostStream
+// priority -1: one thread per listed cell x_b, run on the streamed
+// populations.  For i = 1..18 in order, q = BOUZIDI_DISTANCE[i]:
+//   q > 0:  x_b[opp i] = (q <= 1/2) (2q x_s[i] + (1 - 2q) x_b[i])
+//                      + (q >  1/2) (x_s[i] / 2q + (2q - 1)/2q x_f[opp i])
+//           with x_s = x_b + c_i (solid side) and x_f = x_b - c_i;
+//   q == 0: x_b[opp i] = x_b[i] (full-way bounce-back).
+// The cell's own populations stay in registers, so a later direction reads
+// what an earlier one wrote, as in the reference's loop.  The neighbours'
+// values read here (x_s[i], x_f[opp i]) streamed from x_b itself, so the
+// ring slots of padding cells hold them without an overlap exchange.  The
+// host rejects lists in which one cell's update reads a population another
+// cell's update writes, so the cells can be updated concurrently.
+template <typename S>
+__global__ void __launch_bounds__(256) k_bouzidi(const StepArgs<typename Storage<S>::A> a, const uint32_t* cells,
+                                                 const typename Storage<S>::A* q, uint32_t ncells)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) {
+    return;
+  }
+  const uint32_t n = cells[t];
+  const int64_t step[3] = {(int64_t)a.py * a.pz, (int64_t)a.pz, 1};
+  A f[Q];
+  load_cell<S>(a, n, f);
+  uint32_t written = 0;
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    constexpr int half = 9;
+    const int o = i <= half ? i + half : i - half;
+    const A qi = q[(size_t)(i - 1) * ncells + t];
+    if (qi > A(0)) {
+      const int64_t d = CX(i) * step[0] + CY(i) * step[1] + CZ(i) * step[2];
+      const A xs = Storage<S>::load(ring_at<S>(a.pop, a.stride, i, ring_slot((uint32_t)((int64_t)n + d),
+                                                                              a.shift[i], a.count)));
+      if (qi <= A(0.5)) {
+        f[o] = A(2) * qi * xs + (A(1) - A(2) * qi) * f[i];
+      } else {
+        const A xf = Storage<S>::load(ring_at<S>(a.pop, a.stride, o, ring_slot((uint32_t)((int64_t)n - d),
+                                                                                a.shift[o], a.count)));
+        f[o] = A(0.5) / qi * xs + A(0.5) * (A(2) * qi - A(1)) / qi * xf;
+      }
+      written |= 1u << o;
+    } else if (qi == A(0)) {
+      f[o] = f[i];
+      written |= 1u << o;
+    }
+  }
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    if (written & (1u << i)) {
+      *ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)) = Storage<S>::pack(f[i]);
+    }
   }
 }
 
@@ -1131,32 +1224,6 @@ __global__ void k_ini_equilibrium(S* pop, uint64_t stride, uint32_t count, const
   }
 }
 
-// density and velocity per padded cell
-template <typename S, typename H>
-__global__ void k_rho_u(const S* pop, uint64_t stride, uint32_t count, const uint32_t* shift,
-                        H* rho_out, H* u_out)
-{
-  using A = typename Storage<S>::A;
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= count) {
-    return;
-  }
-  A f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    uint32_t s = n + shift[i];
-    s = s >= count ? s - count : s;
-    f[i] = Storage<S>::unpack(pop[(uint64_t)i * stride + s]);
-  }
-  A drho, jx, jy, jz;
-  moments(f, drho, jx, jy, jz);
-  const A rho = drho + A(1);
-  rho_out[n] = (H)rho;
-  u_out[3 * (size_t)n + 0] = (H)(jx / rho);
-  u_out[3 * (size_t)n + 1] = (H)(jy / rho);
-  u_out[3 * (size_t)n + 2] = (H)(jz / rho);
-}
-
 // FP64 moments of every colliding core cell, summed from the stored values
 template <typename S>
 __global__ void __launch_bounds__(256) k_moments(const S* pop, uint64_t stride, uint32_t count,
@@ -1277,6 +1344,11 @@ struct olbgfx_lattice {
   uint32_t* post_dev = nullptr;
   uint32_t post_edges = 0, post_corners = 0;
   bool boundary_kinds = false;       // some core cell has a LocalVelocity kind
+  // Bouzidi boundaries: the cells that run the post-processor and their
+  // distances, q[(i - 1) * bz_n + k] for direction i of cell k
+  uint32_t* bz_cells = nullptr;
+  void* bz_q = nullptr;
+  uint32_t bz_n = 0;
   uint64_t boundary_version = ~0ull;
   olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
   std::vector<uint8_t> host_mask;
@@ -1425,7 +1497,13 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
       const uint64_t s = (uint64_t)L->shift[i] + (uint64_t)L->dmod[i];
       L->shift[i] = (uint32_t)(s >= L->count ? s - L->count : s);
     }
-    // PostStream: LocalVelocity edges (priority 0), then corners (priority 1)
+    // PostStream: Bouzidi (priority -1), LocalVelocity edges (priority 0),
+    // then corners (priority 1)
+    if (L->bz_n > 0) {
+      for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
+      k_bouzidi<S><<<(L->bz_n + 255) / 256, 256, 0, stream>>>(a, L->bz_cells, (const A*)L->bz_q, L->bz_n);
+      HIP_TRY(hipGetLastError());
+    }
     if (L->post_edges + L->post_corners > 0) {
       for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
       if (L->post_edges) {
@@ -1737,6 +1815,8 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->cell_force);
   hipFree(L->normal_dev);
   hipFree(L->post_dev);
+  hipFree(L->bz_cells);
+  hipFree(L->bz_q);
   hipFree(L->stats_partial);
   hipFree(L->stats_out);
   delete L;
@@ -1843,6 +1923,99 @@ olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* L, const int8_t* normals)
     HIP_TRY(hipMalloc(&L->normal_dev, L->count));
   }
   HIP_TRY(hipMemcpy(L->normal_dev, codes.data(), L->count, hipMemcpyHostToDevice));
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* L, int64_t n, const int64_t* cells, int32_t vt,
+                                 const void* q)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (n < 0 || (n > 0 && (!cells || !q))) {
+    return fail(OLBGFX_ERR_INVALID, "null argument or negative count");
+  }
+  if (vt != OLBGFX_VALUE_F32 && vt != OLBGFX_VALUE_F64) {
+    return fail(OLBGFX_ERR_INVALID, "unknown value type");
+  }
+  if (n > (int64_t)L->count) {
+    return fail(OLBGFX_ERR_INVALID, "more Bouzidi cells than cells");
+  }
+  const olbgfx_lattice_desc& D = L->desc;
+  const int64_t P[3] = {D.nx + 2 * D.padding, D.ny + 2 * D.padding, D.nz + 2 * D.padding};
+  std::vector<double> qd(Q * (size_t)n);
+  for (size_t k = 0; k < qd.size(); ++k) {
+    qd[k] = vt == OLBGFX_VALUE_F64 ? ((const double*)q)[k] : (double)((const float*)q)[k];
+    if (!std::isfinite(qd[k])) {
+      return fail(OLBGFX_ERR_INVALID, "Bouzidi distances must be finite");
+    }
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t c = cells[k];
+    if (c < 0 || c >= (int64_t)L->count || (k > 0 && c <= cells[k - 1])) {
+      return fail(OLBGFX_ERR_INVALID, "Bouzidi cells must be valid ids in ascending order");
+    }
+    const int64_t r[3] = {c / ((int64_t)L->py * L->pz), (c / L->pz) % L->py, c % L->pz};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (r[ax] < D.padding || r[ax] >= P[ax] - D.padding) {
+        return fail(OLBGFX_ERR_INVALID, "Bouzidi cells must be core cells");
+      }
+    }
+  }
+  // a cell's update reads x_s[i] (x_s = x_b + c_i) and, for q > 1/2,
+  // x_f[opp i] (x_f = x_b - c_i); neither may be written by another update
+  auto entry = [&](int64_t c) -> int64_t {
+    const int64_t* it = std::lower_bound(cells, cells + n, c);
+    return (it != cells + n && *it == c) ? it - cells : -1;
+  };
+  for (int64_t k = 0; k < n; ++k) {
+    for (int i = 1; i < Q; ++i) {
+      const double qi = qd[Q * k + i];
+      if (qi < 0) {
+        continue;
+      }
+      const int o = i <= 9 ? i + 9 : i - 9;
+      const int64_t ys = entry(cells[k] + L->d[i]);
+      if (ys >= 0 && qd[Q * ys + o] >= 0) {
+        return fail(OLBGFX_ERR_INVALID, "a Bouzidi update reads a population another one writes");
+      }
+      if (qi > 0.5) {
+        const int64_t yf = entry(cells[k] - L->d[i]);
+        if (yf >= 0 && qd[Q * yf + i] >= 0) {
+          return fail(OLBGFX_ERR_INVALID, "a Bouzidi update reads a population another one writes");
+        }
+      }
+    }
+  }
+  ++L->version;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipFree(L->bz_cells));
+  HIP_TRY(hipFree(L->bz_q));
+  L->bz_cells = nullptr;
+  L->bz_q = nullptr;
+  L->bz_n = 0;
+  if (n == 0) {
+    return OLBGFX_OK;
+  }
+  std::vector<uint32_t> ids(cells, cells + n);
+  // direction-major on the device so the threads of a wavefront read q coalesced
+  const size_t as = is_f64(L) ? 8 : 4;
+  std::vector<char> qa(as * 18 * (size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    for (int i = 1; i < Q; ++i) {
+      const size_t at = (size_t)(i - 1) * n + k;
+      if (as == 8) {
+        ((double*)qa.data())[at] = qd[Q * k + i];
+      } else {
+        ((float*)qa.data())[at] = (float)qd[Q * k + i];
+      }
+    }
+  }
+  HIP_TRY(hipMalloc(&L->bz_cells, sizeof(uint32_t) * ids.size()));
+  HIP_TRY(hipMalloc(&L->bz_q, qa.size()));
+  HIP_TRY(hipMemcpy(L->bz_cells, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(L->bz_q, qa.data(), qa.size(), hipMemcpyHostToDevice));
+  L->bz_n = (uint32_t)n;
   return OLBGFX_OK;
 }
 
@@ -2049,17 +2222,18 @@ olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* L, int32_t vt, void* rho, voi
     hipFree(d_rho);
     return fail(OLBGFX_ERR_NOMEM, "temporary allocation failed");
   }
-  olbgfx_status st = sync_shifts(L);
+  olbgfx_status st = prepare_boundary(L);
   if (st == OLBGFX_OK) {
     const dim3 block(256), grid((L->count + 255) / 256);
     st = dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
       using S = std::remove_pointer_t<decltype(tag)>;
+      using A = typename Storage<S>::A;
+      StepArgs<A> a;
+      fill_args(L, a, false);
       if (vt == OLBGFX_VALUE_F32) {
-        k_rho_u<S, float><<<grid, block>>>((const S*)L->pop, L->stride, L->count, L->shift_dev,
-                                           (float*)d_rho, (float*)d_u);
+        k_cell_rho_u<S, float><<<grid, block>>>(a, (float*)d_rho, (float*)d_u);
       } else {
-        k_rho_u<S, double><<<grid, block>>>((const S*)L->pop, L->stride, L->count, L->shift_dev,
-                                            (double*)d_rho, (double*)d_u);
+        k_cell_rho_u<S, double><<<grid, block>>>(a, (double*)d_rho, (double*)d_u);
       }
       HIP_TRY(hipGetLastError());
       return OLBGFX_OK;
@@ -2210,6 +2384,9 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
     }
     if (L->boundary_kinds) {
       return fail(OLBGFX_ERR_UNSUPPORTED, "ensembles do not take members with LocalVelocity boundaries");
+    }
+    if (L->bz_n > 0) {
+      return fail(OLBGFX_ERR_UNSUPPORTED, "ensembles do not take members with Bouzidi boundaries");
     }
   }
   bool changed = force || E->tiled_block != E->block_threads || E->grouped_bytes != E->group_bytes;

@@ -9,9 +9,13 @@
  *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
  *            [--STORAGE f32|f16|bf16] [--DUMP file] [--TUNE_CACHE file]
  *            [--BOUNDARY local_velocity|bounce_back] [--SAVE file] [--LOAD file]
+ *            [--VTK dir]
  *
  * --DUMP writes the materials and the populations of the padded block after
- * the run (logical order, float32) for the tests.
+ * the run (logical order, float32) for the tests.  --VTK writes velocity,
+ * density, physVelocity and physPressure after the run with SuperVTMwriter3D
+ * into dir (cuboid of length 1, spacing 1/RESOLUTION, lattice-unit factors),
+ * as the reference's getResults does with its own writer.
  */
 #include <chrono>
 #include <cstdio>
@@ -21,6 +25,7 @@
 #include <string>
 
 #include "olb_gfx950.h"
+#include "olb_gfx950_vtk.h"
 
 using namespace olb;
 using T = float;
@@ -37,6 +42,7 @@ struct Parameters {
   bool localVelocity = true;  // cavity3d.cpp:111-124: LID_DRIVEN without LID_DRIVEN_BOUNCE_BACK
   std::string save;           // checkpoint after the timed steps
   std::string load;           // checkpoint to restart from before the warm-up
+  std::string vtk;            // VTK output directory
 };
 
 static Parameters fromCLI(int argc, char** argv)
@@ -52,6 +58,7 @@ static Parameters fromCLI(int argc, char** argv)
     else if (k == "--TUNE_CACHE") p.tuneCache = v;
     else if (k == "--SAVE") p.save = v;
     else if (k == "--LOAD") p.load = v;
+    else if (k == "--VTK") p.vtk = v;
     else if (k == "--BOUNDARY") {
       if (v != "local_velocity" && v != "bounce_back") {
         throw std::invalid_argument("--BOUNDARY is local_velocity or bounce_back");
@@ -134,6 +141,16 @@ int main(int argc, char** argv)
 
     if (!p.save.empty()) {
       lattice.save(p.save);
+    }
+    if (!p.vtk.empty()) {
+      geometry.setPhysical({T(0), T(0), T(0)}, T(1) / T(n));
+      SuperVTMwriter3D<T> vtm(p.localVelocity ? "cavity_lv" : "cavity", p.vtk);
+      vtm.addFunctor(VtkFunctor::Velocity);
+      vtm.addFunctor(VtkFunctor::Density);
+      vtm.addFunctor(VtkFunctor::PhysVelocity);
+      vtm.addFunctor(VtkFunctor::PhysPressure);
+      vtm.createMasterFile();
+      vtm.write(lattice, 10 + p.timeSteps);
     }
     lattice.setProcessingContext(ProcessingContext::Evaluation);
     std::cout << p.resolution << ", " << p.timeSteps << ", 1, 1, 1, " << mlups << std::endl;

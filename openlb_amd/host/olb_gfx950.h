@@ -13,6 +13,8 @@
  *   setStatisticsOn/Off, getStatistics         core/superLattice.h, core/latticeStatistics.h:43-127
  *   setProcessingContext(Evaluation)           core/superLattice.hh:184-195
  *   save / load (OpenLB checkpoint format)     core/serializer.h:181-189, io/serializerIO.hh:40-64
+ *   setBouzidiBoundary<..., BouzidiPostProcessor> boundary/setBouzidiBoundary.h:193-330
+ *   IndicatorCylinder3D, IndicatorF3D::distance functors/analytical/indicator/indicatorF3D.hh:312-418
  *   autotune (an addition of this platform)    no counterpart; OpenLB fixes launch shapes
  *
  * Everything below calls the C ABI of include/olbgfx.h; no HIP header is
@@ -35,6 +37,7 @@
 #include <fstream>
 #include <functional>
 #include <iterator>
+#include <limits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -82,6 +85,7 @@ struct D3Q19 {
     {1, 0, -1}, {0, 1, 1}, {0, 1, -1}};
   static constexpr int tden[19] = {3, 18, 18, 18, 36, 36, 36, 36, 36, 36,
                                    18, 18, 18, 36, 36, 36, 36, 36, 36};
+  static constexpr int opposite(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
 };
 
 struct OMEGA { };
@@ -155,6 +159,21 @@ public:
 
   std::array<int, 3> getExtent() const { return _extent; }
   int getOverlap() const { return _overlap; }
+  /// physical placement of the cuboid: origin of core cell (0, 0, 0) and the
+  /// spacing (geometry/cuboid.h:90-92); lattice units by default
+  void setPhysical(std::array<T, 3> origin, T deltaR)
+  {
+    _origin = origin;
+    _deltaR = deltaR;
+  }
+  T getDeltaR() const { return _deltaR; }
+  std::array<T, 3> getPhysR(int x, int y, int z) const
+  {
+    const int r[3] = {x, y, z};
+    std::array<T, 3> p;
+    for (int a = 0; a < 3; ++a) p[a] = _origin[a] + T(r[a]) * _deltaR;
+    return p;
+  }
   std::array<bool, 3> getPeriodicity() const { return _periodic; }
   std::size_t getNcells() const { return _material.size(); }
 
@@ -229,6 +248,8 @@ private:
   std::array<int, 3> _extent;
   int _overlap;
   std::array<bool, 3> _periodic;
+  std::array<T, 3> _origin{T(0), T(0), T(0)};
+  T _deltaR = T(1);
   std::array<int, 3> _size{};
   std::vector<std::uint8_t> _material;
 };
@@ -441,6 +462,18 @@ public:
     _dirty = true;
   }
 
+  /// the cells that run BouzidiPostProcessor and their 19 distances each
+  /// (setBouzidiBoundary below computes them); replaces any earlier list
+  void setBouzidi(std::vector<std::int64_t> cells, std::vector<double> q)
+  {
+    if (q.size() != (std::size_t)DESCRIPTOR::q * cells.size()) {
+      throw std::invalid_argument("19 Bouzidi distances per cell");
+    }
+    _bouzidiCells = std::move(cells);
+    _bouzidiQ = std::move(q);
+    _dirty = true;
+  }
+
   /// set a dynamics parameter on every dynamics that has it
   template <typename PARAMETER>
   void setParameter(T value)
@@ -518,6 +551,8 @@ public:
     }
     check(olbgfx_set_dynamics_mask(handle(), _slotOfCell.data()));
     check(olbgfx_set_cell_normals(handle(), _normals.empty() ? nullptr : _normals.data()));
+    check(olbgfx_set_bouzidi(handle(), (int64_t)_bouzidiCells.size(), _bouzidiCells.data(), OLBGFX_VALUE_F64,
+                             _bouzidiQ.data()));
     for (auto& kv : _fields) {
       check(olbgfx_set_cell_field(handle(), kv.first, valueType(),
                                   kv.second.uniform ? nullptr : kv.second.values.data()));
@@ -651,6 +686,20 @@ public:
     _hostValid = true;
     setProcessingContext(ProcessingContext::Simulation);
   }
+
+  /// density and velocity of every padded cell through the momenta of its
+  /// dynamics (Cell::computeRhoU), computed on the device
+  void computeRhoU(std::vector<T>& rho, std::vector<T>& u)
+  {
+    if (!_initialized || _dirty) {
+      initialize();
+    }
+    rho.resize(_ncells);
+    u.resize(3 * _ncells);
+    check(olbgfx_compute_rho_u(handle(), valueType(), rho.data(), u.data()));
+  }
+
+  SuperGeometry<T>& getGeometry() { return _geometry; }
 
   /// population iPop of cell iCell, shifted by -t_i (evaluation mirror)
   T& population(CellID iCell, int iPop)
@@ -791,6 +840,8 @@ private:
   std::vector<Slot> _slots;
   std::vector<std::uint8_t> _slotOfCell;
   std::vector<std::int8_t> _normals;  // discrete normal per cell for LocalVelocity
+  std::vector<std::int64_t> _bouzidiCells;
+  std::vector<double> _bouzidiQ;
   std::map<int32_t, Field> _fields;
   std::vector<T> _host;
   bool _hostValid = false;
@@ -826,6 +877,261 @@ void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int m
   }
 }
 }  // namespace boundary
+
+// ---- Bouzidi boundaries (boundary/setBouzidiBoundary.h:193-330) ------------
+
+/// analytical indicator: inside test, bounding box and the ray distance of
+/// IndicatorF3D (functors/analytical/indicator/indicatorBaseF3D.hh:57-87,
+/// the bisection of indicatorBase.h:35-93 with precision 1e-4, pitch 1/2)
+template <typename S>
+class IndicatorF3D {
+public:
+  virtual ~IndicatorF3D() = default;
+  virtual bool operator()(bool output[1], const S input[3]) = 0;
+  const std::array<S, 3>& getMin() const { return _myMin; }
+  const std::array<S, 3>& getMax() const { return _myMax; }
+  bool isInsideBox(const std::array<S, 3>& p) const
+  {
+    bool in = true;
+    for (int a = 0; a < 3; ++a) in = in && _myMin[a] <= p[a] && p[a] <= _myMax[a];
+    return in;
+  }
+  /// distance from origin along direction to the first change of inside /
+  /// outside; false if the ray leaves the bounding box from outside first
+  virtual bool distance(S& distance, const std::array<S, 3>& origin, const std::array<S, 3>& direction,
+                        int iC = -1)
+  {
+    (void)iC;
+    const S precision = S(.0001);
+    S pitch = S(0.5);
+    auto at = [](const std::array<S, 3>& o, S t, const std::array<S, 3>& d) {
+      return std::array<S, 3>{o[0] + t * d[0], o[1] + t * d[1], o[2] + t * d[2]};
+    };
+    auto inside = [this](const std::array<S, 3>& p) {
+      bool v = false;
+      (*this)(&v, p.data());
+      return v;
+    };
+    const bool nearIn = inside(at(origin, 10 * precision, direction));
+    const bool nearOut = inside(at(origin, -(10 * precision), direction));
+    if (nearIn == !nearOut) {
+      distance = 0;
+      return true;
+    }
+    const bool originValue = inside(origin);
+    std::array<S, 3> cur = origin;
+    bool value = originValue;
+    while (value == originValue && isInsideBox(cur)) {
+      for (int a = 0; a < 3; ++a) cur[a] += direction[a];
+      value = inside(cur);
+    }
+    if (!isInsideBox(cur) && !originValue) {
+      return false;
+    }
+    while (pitch >= precision) {
+      if (!isInsideBox(cur) && originValue) {
+        for (int a = 0; a < 3; ++a) cur[a] -= pitch * direction[a];
+      } else {
+        value = inside(cur);
+        const S sign = value == originValue ? S(1) : S(-1);
+        for (int a = 0; a < 3; ++a) cur[a] += sign * pitch * direction[a];
+      }
+      pitch *= S(0.5);
+    }
+    S n2 = 0;
+    for (int a = 0; a < 3; ++a) n2 += (cur[a] - origin[a]) * (cur[a] - origin[a]);
+    distance = std::sqrt(n2);
+    return true;
+  }
+
+protected:
+  std::array<S, 3> _myMin{}, _myMax{};
+};
+
+/// util::epsRelative (utilities/floatingPointRelationalOperators.h:278-296)
+template <typename S>
+S epsRelative(S a)
+{
+  const S e = std::numeric_limits<S>::epsilon();
+  if (a == S() || (a > -e && a < e)) {
+    return std::numeric_limits<S>::min() / e;
+  }
+  return S(3) * e * std::fabs(a);
+}
+
+/// IndicatorCylinder3D(center1, center2, radius)
+/// (functors/analytical/indicator/indicatorF3D.hh:312-418)
+template <typename S>
+class IndicatorCylinder3D : public IndicatorF3D<S> {
+public:
+  IndicatorCylinder3D(std::array<S, 3> center1, std::array<S, 3> center2, S radius)
+    : _c1(center1), _c2(center2), _radius2(radius * radius)
+  {
+    S baba = 0;
+    for (int a = 0; a < 3; ++a) baba += (_c2[a] - _c1[a]) * (_c2[a] - _c1[a]);
+    const S length = std::sqrt(baba);
+    std::array<S, 3> K, I, J;
+    for (int a = 0; a < 3; ++a) K[a] = (_c2[a] - _c1[a]) / length;
+    if (_c2[1] == _c1[1] && _c2[0] == _c1[0]) {
+      I = {1, 0, 0};
+      J = {0, 1, 0};
+    } else {
+      const S normi = std::sqrt(K[1] * K[1] + K[0] * K[0]);
+      I = {-K[1] / normi, K[0] / normi, 0};
+      J = {K[1] * I[2] - K[2] * I[1], K[2] * I[0] - K[0] * I[2], K[0] * I[1] - K[1] * I[0]};
+    }
+    const S r = std::sqrt(_radius2);
+    for (int a = 0; a < 3; ++a) {
+      const S w = std::sqrt(I[a] * I[a] + J[a] * J[a]) * r;
+      this->_myMax[a] = _c1[a] + w + std::max(K[a] * length, S(0));
+      this->_myMin[a] = _c1[a] - w + std::min(K[a] * length, S(0));
+    }
+    _epsMax = std::max(epsRelative(this->_myMax[0]),
+                       std::max(epsRelative(this->_myMax[1]), epsRelative(this->_myMax[2])));
+    _epsRad = epsRelative(std::sqrt(_radius2));
+  }
+
+  bool operator()(bool output[1], const S input[3]) override
+  {
+    S pa[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+      pa[a] = input[a] - _c1[a];
+      d[a] = _c2[a] - _c1[a];
+    }
+    const S h = (pa[0] * d[0] + pa[1] * d[1] + pa[2] * d[2]) / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (h < S(0) - _epsMax || h > S(1) + _epsMax) {
+      output[0] = false;
+      return true;
+    }
+    S dist2 = 0;
+    for (int a = 0; a < 3; ++a) {
+      const S r = input[a] - (_c1[a] + h * d[a]);
+      dist2 += r * r;
+    }
+    output[0] = dist2 < _radius2 + _epsRad;
+    return true;
+  }
+
+private:
+  std::array<S, 3> _c1, _c2;
+  S _radius2;
+  S _epsMax = 0, _epsRad = 0;
+};
+
+/// BOUZIDI_DISTANCE field (19 per padded cell, -1 where unset) that
+/// setBouzidiBoundary (setBouzidiBoundary.h:229-330) writes for the solid
+/// material and the bulk materials: for every solid cell of the padded block
+/// and every direction i with x_b = x_s + c_i inside the block, q[opp i] of
+/// x_b is the ray distance along -c_i normed by |c_i| dx (0.5 if not found or
+/// outside [0, 1], or if x_b + c_i is not bulk) when x_b is bulk, and 0.5
+/// when x_b is neither bulk nor material 0
+template <typename T>
+std::vector<double> bouzidiDistances(const SuperGeometry<T>& geometry, int material, IndicatorF3D<T>& indicator,
+                                     const std::vector<int>& bulkMaterials = {1})
+{
+  using D = descriptors::D3Q19;
+  const auto e = geometry.getExtent();
+  const int p = geometry.getOverlap();
+  std::vector<double> q((std::size_t)D::q * geometry.getNcells(), -1.0);
+  auto bulk = [&](int x, int y, int z) {
+    const int m = geometry.getOrOutside(x, y, z);
+    return std::find(bulkMaterials.begin(), bulkMaterials.end(), m) != bulkMaterials.end();
+  };
+  auto inside = [&](int x, int y, int z) {
+    return x >= -p && y >= -p && z >= -p && x < e[0] + p && y < e[1] + p && z < e[2] + p;
+  };
+  const T dx = geometry.getDeltaR();
+  for (int x = -p; x < e[0] + p; ++x) {
+    for (int y = -p; y < e[1] + p; ++y) {
+      for (int z = -p; z < e[2] + p; ++z) {
+        if (geometry.get(x, y, z) != material) continue;
+        for (int i = 1; i < D::q; ++i) {
+          const int bx = x + D::c[i][0], by = y + D::c[i][1], bz = z + D::c[i][2];
+          if (!inside(bx, by, bz)) continue;
+          const std::size_t at = (std::size_t)D::opposite(i) * geometry.getNcells() + geometry.getCellId(bx, by, bz);
+          if (bulk(bx, by, bz)) {
+            const std::array<T, 3> origin = geometry.getPhysR(bx, by, bz);
+            const std::array<T, 3> direction = {-dx * T(D::c[i][0]), -dx * T(D::c[i][1]), -dx * T(D::c[i][2])};
+            const T norm = dx * std::sqrt(T(D::c[i][0] * D::c[i][0] + D::c[i][1] * D::c[i][1] +
+                                            D::c[i][2] * D::c[i][2]));
+            T dist = -1;
+            T qi = 0.5;
+            if (indicator.distance(dist, origin, direction)) {
+              qi = dist / norm;
+              if (qi < 0 || qi > 1) qi = 0.5;
+            }
+            q[at] = bulk(bx + D::c[i][0], by + D::c[i][1], bz + D::c[i][2]) ? (double)qi : 0.5;
+          } else if (geometry.getOrOutside(bx, by, bz) != 0) {
+            q[at] = 0.5;
+          }
+        }
+      }
+    }
+  }
+  return q;
+}
+
+/// setBouzidiBoundary<T, DESCRIPTOR, BouzidiPostProcessor>(lattice, geometry,
+/// material, indicator, bulkMaterials): the non-padding cells with an entry
+/// run the post-processor after each stream.  The reference applies them in
+/// ascending cell id; the platform updates them concurrently, which differs
+/// only for q = 1/2 copies between two solid cells that read each other's
+/// output -- there the lower id takes the other's value and the higher id's
+/// copy would write back what it holds, so it is dropped here.
+template <typename T, typename DESCRIPTOR>
+void setBouzidiBoundary(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material,
+                        IndicatorF3D<T>& indicator, std::vector<int> bulkMaterials = {1})
+{
+  using D = descriptors::D3Q19;
+  const std::vector<double> field = bouzidiDistances(geometry, material, indicator, bulkMaterials);
+  const std::size_t n = geometry.getNcells();
+  const auto e = geometry.getExtent();
+  std::vector<std::int64_t> cells;
+  std::vector<std::int64_t> listed(n, -1);
+  geometry.forCore([&](int x, int y, int z) {
+    const CellID c = geometry.getCellId(x, y, z);
+    for (int i = 1; i < D::q; ++i) {
+      if (field[(std::size_t)i * n + c] >= 0) {
+        listed[c] = (std::int64_t)cells.size();
+        cells.push_back(c);
+        return;
+      }
+    }
+  });
+  (void)e;
+  std::sort(cells.begin(), cells.end());
+  for (std::size_t k = 0; k < cells.size(); ++k) listed[(std::size_t)cells[k]] = (std::int64_t)k;
+  std::vector<double> q(D::q * cells.size());
+  for (std::size_t k = 0; k < cells.size(); ++k) {
+    for (int i = 0; i < D::q; ++i) q[D::q * k + i] = field[(std::size_t)i * n + (std::size_t)cells[k]];
+  }
+  const std::int64_t step[3] = {(std::int64_t)(e[1] + 2 * geometry.getOverlap()) * (e[2] + 2 * geometry.getOverlap()),
+                                (std::int64_t)(e[2] + 2 * geometry.getOverlap()), 1};
+  for (std::size_t k = 0; k < cells.size(); ++k) {
+    for (int j = 1; j < D::q; ++j) {
+      if (q[D::q * k + j] < 0) continue;
+      const std::int64_t dj = D::c[j][0] * step[0] + D::c[j][1] * step[1] + D::c[j][2] * step[2];
+      const std::int64_t y = listed[(std::size_t)(cells[k] + dj)];
+      const int o = D::opposite(j);
+      if (y >= 0 && q[D::q * (std::size_t)y + o] >= 0) {
+        if (q[D::q * k + j] != 0.5 || q[D::q * (std::size_t)y + o] != 0.5) {
+          throw std::invalid_argument("two Bouzidi cells read each other's updates");
+        }
+        if ((std::size_t)y < k) q[D::q * k + j] = -1.0;
+      }
+    }
+  }
+  std::vector<std::int64_t> keptCells;
+  std::vector<double> keptQ;
+  for (std::size_t k = 0; k < cells.size(); ++k) {
+    bool any = false;
+    for (int i = 1; i < D::q; ++i) any = any || q[D::q * k + i] >= 0;
+    if (!any) continue;
+    keptCells.push_back(cells[k]);
+    keptQ.insert(keptQ.end(), q.begin() + (std::ptrdiff_t)(D::q * k), q.begin() + (std::ptrdiff_t)(D::q * (k + 1)));
+  }
+  lattice.setBouzidi(std::move(keptCells), std::move(keptQ));
+}
 
 }  // namespace olb
 

@@ -136,6 +136,9 @@ typedef struct {
   real* cell_rho;       /* optional per-cell RHO (count) */
   real* cell_force;     /* optional per-cell FORCE (count*3) */
   int8_t* normals;      /* optional per-cell discrete normal (count*3) */
+  int64_t bz_n;         /* cells running BouzidiPostProcessor */
+  int64_t* bz_cells;    /* their ids, in the order they are applied */
+  real* bz_q;           /* their BOUZIDI_DISTANCE, 19 per cell */
   int store;
   int stats_on;
   double stats[4];      /* nCells, sum rho, sum uSqr, max uSqr (last step) */
@@ -284,6 +287,8 @@ void ORC(destroy)(void* h)
   free(L->cell_rho);
   free(L->cell_force);
   free(L->normals);
+  free(L->bz_cells);
+  free(L->bz_q);
   free(L);
 }
 
@@ -848,9 +853,40 @@ static void post_corner(lattice_t* L, uint64_t n, const int N[3])
   regularise(L, n, rho, du);
 }
 
-/* PostStream stage: edges (priority 0) before corners (priority 1) */
+/* BouzidiPostProcessor::apply (boundary/setBouzidiBoundary.h:48-80) on every
+ * listed cell in the listed order (the reference's ascending cell id,
+ * core/platform/cpu/sisd/operator.h:368-386), in place.  Neighbours are read
+ * from their own ring slots: the values read (x_s[i], x_f[opp i] and x_b[i])
+ * all streamed from a core cell, so a padding cell's slot already holds what
+ * the reference's PostStream overlap exchange would put there. */
+static void post_bouzidi(lattice_t* L)
+{
+  for (int64_t e = 0; e < L->bz_n; ++e) {
+    const uint64_t n = (uint64_t)L->bz_cells[e];
+    const real* q = &L->bz_q[Q * e];
+    for (int i = 1; i < Q; ++i) {
+      const int o = opp(i);
+      real* xb_o = &L->ring[o][slot(L, o, n)];
+      const real xb_i = L->ring[i][slot(L, i, n)];
+      if (q[i] > (real)0) {
+        const int64_t dn = L->d[i];
+        const real xs = L->ring[i][slot(L, i, (uint64_t)((int64_t)n + dn))];
+        const real xf = L->ring[o][slot(L, o, (uint64_t)((int64_t)n - dn))];
+        const real lo = (real)(q[i] <= (real)0.5), hi = (real)(q[i] > (real)0.5);
+        *xb_o = orc_quantize(L, lo * ((real)2 * q[i] * xs + ((real)1 - (real)2 * q[i]) * xb_i)
+                              + hi * ((real)0.5 / q[i] * xs + (real)0.5 * ((real)2 * q[i] - (real)1) / q[i] * xf));
+      } else if (q[i] == (real)0) {
+        *xb_o = xb_i;
+      }
+    }
+  }
+}
+
+/* PostStream stage: Bouzidi (priority -1), LocalVelocity edges (priority 0),
+ * corners (priority 1) */
 static void post_stream(lattice_t* L)
 {
+  post_bouzidi(L);
   if (!L->normals) return;
   for (int pass = 2; pass <= 3; ++pass) {
     for (int x = 0; x < L->nx; ++x) {
@@ -877,6 +913,24 @@ void ORC(set_cell_normals)(void* h, const int8_t* normals)
     L->normals = (int8_t*)malloc(3 * L->count);
     memcpy(L->normals, normals, 3 * L->count);
   }
+}
+
+/* the cells that run BouzidiPostProcessor and their BOUZIDI_DISTANCE
+ * (19 per cell); n = 0 removes them */
+void ORC(set_bouzidi)(void* h, int64_t n, const int64_t* cells, const real* q)
+{
+  lattice_t* L = (lattice_t*)h;
+  free(L->bz_cells);
+  free(L->bz_q);
+  L->bz_cells = NULL;
+  L->bz_q = NULL;
+  L->bz_n = 0;
+  if (n <= 0) return;
+  L->bz_cells = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+  L->bz_q = (real*)malloc(sizeof(real) * Q * (size_t)n);
+  memcpy(L->bz_cells, cells, sizeof(int64_t) * (size_t)n);
+  memcpy(L->bz_q, q, sizeof(real) * Q * (size_t)n);
+  L->bz_n = n;
 }
 
 void ORC(collide_and_stream)(void* h, int steps)

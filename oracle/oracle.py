@@ -50,6 +50,7 @@ def library() -> ctypes.CDLL:
             g("set_cell_rho").argtypes = [vp, vp]
             g("set_cell_force").argtypes = [vp, vp]
             g("set_cell_normals").argtypes = [vp, vp]
+            g("set_bouzidi").argtypes = [vp, ctypes.c_int64, vp, vp]
             g("get_populations").argtypes = [vp, vp]
             g("set_populations").argtypes = [vp, vp]
             g("collide_and_stream").argtypes = [vp, i32]
@@ -157,6 +158,14 @@ class OracleLattice:
         v = np.ascontiguousarray(normals, dtype=np.int8).reshape(-1)
         assert v.size == 3 * self.ncells
         self._f("set_cell_normals")(self._h, _ptr(v))
+
+    def set_bouzidi(self, cells, q) -> None:
+        """the cells running BouzidiPostProcessor (applied in the given order)
+        and their BOUZIDI_DISTANCE (n x 19); an empty list removes them"""
+        cells = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=self._dtype).reshape(-1)
+        assert q.size == Q * cells.size
+        self._f("set_bouzidi")(self._h, int(cells.size), _ptr(cells), _ptr(q))
 
     def ini_equilibrium(self, rho, u, sel=None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)

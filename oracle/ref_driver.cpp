@@ -14,12 +14,17 @@
  *                                                BounceBack walls + BounceBackVelocity lid
  *   olb_ref cavity_lv RES STEPS ULID RE OUT       the same with LocalVelocity walls and lid
  *   olb_ref tgv      N STEPS U0 TAU OUT           periodic Taylor-Green, BGK
+ *   olb_ref vtk_tgv  N STEPS OUT DIR              the same (u0 0.04, tau 0.56), then SuperVTMwriter3D
+ *                                                output into DIR/vtkData (cavity modes: optional DIR)
  *   olb_ref tgv_smagorinsky N STEPS U0 TAU CS OUT  the same with SmagorinskyBGKdynamics
  *   olb_ref porous   NX NY NZ SPHERES R TAU MAGIC FX STEPS OUT
  *                                                periodic sphere pack, ForcedTRT + BounceBack
  *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
  *                                                convergence (the reference's KAT)
  *   olb_ref poiseuille_bouzidi N OUT              the same with the Bouzidi wall (its bouzidi row)
+ *   olb_ref poiseuille_local N OUT                its forced local row (LocalVelocity wall)
+ *   olb_ref poiseuille_local_nonforced N OUT      its non-forced local row (LocalVelocity wall and
+ *                                                inflow, LocalPressure outflow)
  *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
  *   olb_ref checkpoint N STEPS OUT NAME           TGV after STEPS steps: populations dump and
  *                                                OpenLB's own checkpoint (SuperLattice::save)
@@ -158,7 +163,32 @@ void dump_normals(Dump& d, SuperGeometry<T,3>& G, int padding, std::initializer_
   d.array("normals", "i4", normal);
 }
 
-int run_cavity(int res, int steps, double ulid, double re, const std::string& out, bool local_velocity)
+// SuperVTMwriter3D output (io/superVtmWriter3D.hh) of the lattice at iT
+// into DIR/vtkData/: velocity, density, physVelocity and physPressure, once
+// compressed (the default) as NAME and once as plain binary as NAME_raw
+template <typename DESCRIPTOR>
+void write_vtk(SuperLattice<T,DESCRIPTOR>& L, const UnitConverter<T,DESCRIPTOR>& converter,
+               const std::string& dir, const std::string& name, int iT)
+{
+  singleton::directories().setOutputDir(dir + "/");
+  SuperLatticeVelocity3D<T,DESCRIPTOR> velocity(L);
+  SuperLatticeDensity3D<T,DESCRIPTOR> density(L);
+  SuperLatticePhysVelocity3D<T,DESCRIPTOR> physVelocity(L, converter);
+  SuperLatticePhysPressure3D<T,DESCRIPTOR> physPressure(L, converter);
+  SuperVTMwriter3D<T> packed(name);
+  SuperVTMwriter3D<T> raw(name + "_raw", 1, true, false);
+  for (SuperVTMwriter3D<T>* w : {&packed, &raw}) {
+    w->addFunctor(velocity);
+    w->addFunctor(density);
+    w->addFunctor(physVelocity);
+    w->addFunctor(physPressure);
+    w->createMasterFile();
+    w->write(iT);
+  }
+}
+
+int run_cavity(int res, int steps, double ulid, double re, const std::string& out, bool local_velocity,
+               const std::string& vtk = "")
 {
   using DESCRIPTOR = descriptors::D3Q19<>;
   const T length = 1;
@@ -213,6 +243,9 @@ int run_cavity(int res, int steps, double ulid, double re, const std::string& ou
     L.collideAndStream();
   }
   d.array("f1", tname(), populations(L));
+  if (!vtk.empty()) {
+    write_vtk(L, lattice_units<DESCRIPTOR>(nu), vtk, local_velocity ? "cavity_lv" : "cavity", steps);
+  }
   return 0;
 }
 
@@ -376,6 +409,226 @@ int run_porous(int nx, int ny, int nz, const std::string& spheres, double radius
 // ---- Poiseuille pipe, forced BGK + BounceBack, run to convergence: the set-up
 // of the reference's known-answer test tests/benchmarks/testPoiseuille3DBGK.cpp
 // (forced, bounceBack row), driven through the same public API ----
+// Taylor-Green vortex (u0 = 0.04, tau = 0.56) after STEPS steps, dumped and
+// written as VTK through a converter with non-unit factors: dx = 1, dt = 0.5,
+// physical density 1.2 (velocity factor 2, pressure factor 4.8)
+int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir)
+{
+  using DESCRIPTOR = descriptors::D3Q19<>;
+  CuboidDecomposition3D<T>* cuboids;
+  HeuristicLoadBalancer<T>* balancer;
+  SuperGeometry<T,3>* G;
+  build_periodic_box<DESCRIPTOR>(n, n, n, cuboids, balancer, G);
+  const double u0 = 0.04, tau = 0.56;
+  const T nu = T((tau - 0.5) / 3.0);
+  UnitConverter<T,DESCRIPTOR> converter(T(1), T(0.5), T(1), T(2 * u0), T(2) * nu, T(1.2));
+  SuperLattice<T,DESCRIPTOR> L(converter, *G);
+  L.setStatisticsOff();
+  dynamics::set<BGKdynamics>(L, *G, 1);
+  L.setParameter<descriptors::OMEGA>(T(1.0 / tau));
+  TgvField rho(u0, n, true), u(u0, n, false);
+  L.iniEquilibrium(*G, 1, rho, u);
+  L.initialize();
+  Dump d(out, block_header(L) + " case=tgv n=" + std::to_string(n) + " steps=" + std::to_string(steps) +
+                  " u0=" + std::to_string(u0) + " tau=" + std::to_string(tau) + " velocity_factor=" +
+                  std::to_string((double)converter.getConversionFactorVelocity()) + " pressure_factor=" +
+                  std::to_string((double)converter.getConversionFactorPressure()));
+  d.array("materials", "i4", materials(*G, 3));
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), populations(L));
+  for (int t = 0; t < steps; ++t) {
+    L.collideAndStream();
+  }
+  d.array("f1", tname(), populations(L));
+  write_vtk(L, converter, dir, "tgv", steps);
+  return 0;
+}
+
+// the "local" rows of testPoiseuille3DBGK (lines 55-60): LocalVelocity on
+// the pipe wall (and the inflow disc), LocalPressure on the outflow disc when
+// not forced; geometry, dynamics and initial values as the test's
+// prepareGeometry/prepareLattice (lines 72-242).  Dumps, besides populations
+// and energies, the discrete normals of materials 2-4, every cell's
+// computeRhoU after initialize (the velocity and density the boundary momenta
+// hold) and the analytic velocity and pressure of the fluid.
+int run_poiseuille_local(int n, const std::string& out, bool forced)
+{
+  using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
+  const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
+  UnitConverterFromResolutionAndRelaxationTime<T,DESCRIPTOR> const converter(n, T(0.8), T(1), T(1), T(1) / Re, T(1));
+  const T dx = converter.getPhysDeltaX();
+  Vector<T,3> a0(0, r, r), a1(lx, r, r);
+  IndicatorCylinder3D<T> pipe(a0, a1, r);
+  IndicatorLayer3D<T> extended(pipe, dx);
+  CuboidDecomposition3D<T> cuboids(extended, dx, 1);
+  if (forced) {
+    cuboids.setPeriodicity({true, false, false});
+  }
+  HeuristicLoadBalancer<T> balancer(cuboids);
+  SuperGeometry<T,3> G(cuboids, balancer, 2);
+
+  Vector<T,3> c0(-dx * 0.2, r, r), c1(lx, r, r);
+  if (forced) {
+    c0[0] -= 3. * dx;
+    c1[0] += 3. * dx;
+  }
+  IndicatorCylinder3D<T> pipeLong(c0, c1, r);
+  G.rename(0, 2);
+  G.rename(2, 1, pipeLong);
+  if (!forced) {
+    Vector<T,3> o(0, r, r), e = o;
+    o[0] = -dx * 2;
+    e[0] = dx * 2;
+    IndicatorCylinder3D<T> inflow(o, e, r);
+    G.rename(2, 3, 1, inflow);
+    o[0] = lx - 2 * dx;
+    e[0] = lx + 2 * dx;
+    IndicatorCylinder3D<T> outflow(e, o, r);
+    G.rename(2, 4, 1, outflow);
+  }
+  G.clean(false);
+  G.innerClean(false);
+  G.checkForErrors(false);
+
+  SuperLattice<T,DESCRIPTOR> L(converter, G);
+  L.template defineDynamics<NoDynamics>(G, 0);
+  if (forced) {
+    L.template defineDynamics<ForcedBGKdynamics>(G, 1);
+    L.template defineDynamics<ForcedBGKdynamics>(G, 2);
+  } else {
+    L.template defineDynamics<BGKdynamics>(G, 1);
+    L.template defineDynamics<BGKdynamics>(G, 2);
+  }
+  boundary::set<boundary::LocalVelocity>(L, G, 2);
+  if (!forced) {
+    L.template defineDynamics<BGKdynamics>(G, 3);
+    boundary::set<boundary::LocalVelocity>(L, G, 3);
+    L.template defineDynamics<BGKdynamics>(G, 4);
+    boundary::set<boundary::LocalPressure>(L, G, 4);
+  }
+  std::vector<T> axisPoint = {lx, r, r};
+  std::vector<T> axisDirection = {1, 0, 0};
+  CirclePoiseuille3D<T> poiseuilleU(axisPoint, axisDirection, converter.getCharLatticeVelocity(), r);
+  T force0 = 0;
+  if (forced) {
+    const T D = converter.getLatticeLength(diameter);
+    std::vector<T> f(3, T());
+    f[0] = 4. * converter.getLatticeViscosity() * converter.getCharLatticeVelocity() / (D * D / 4.);
+    force0 = f[0];
+    AnalyticalConst3D<T,T> force(f);
+    L.template defineField<descriptors::FORCE>(G, 1, force);
+    L.template defineField<descriptors::FORCE>(G, 2, force);
+    AnalyticalConst3D<T,T> rhoF(1);
+    L.defineRhoU(G, 1, rhoF, poiseuilleU);
+    L.iniEquilibrium(G, 1, rhoF, poiseuilleU);
+    L.defineRhoU(G, 2, rhoF, poiseuilleU);
+    L.iniEquilibrium(G, 2, rhoF, poiseuilleU);
+  } else {
+    T p0 = 4. * converter.getPhysViscosity() * converter.getCharPhysVelocity() * lx / (r * r);
+    p0 = converter.getLatticePressure(p0);
+    AnalyticalLinear3D<T,T> rho(-p0 / lx * descriptors::invCs2<T,DESCRIPTOR>(), 0, 0,
+                                p0 * descriptors::invCs2<T,DESCRIPTOR>() + 1);
+    std::vector<T> velocity(3, T());
+    AnalyticalConst3D<T,T> uF(velocity);
+    L.defineRhoU(G, 0, rho, uF);
+    L.iniEquilibrium(G, 0, rho, uF);
+    for (int m = 1; m <= 4; ++m) {
+      L.defineRhoU(G, m, rho, poiseuilleU);
+      L.iniEquilibrium(G, m, rho, poiseuilleU);
+    }
+  }
+  const T omega = converter.getLatticeRelaxationFrequency();
+  L.template setParameter<descriptors::OMEGA>(omega);
+  L.initialize();
+
+  const int padding = G.getOverlap();
+  std::vector<T> f0 = populations(L);
+  // what the boundary momenta hold: computeRhoU of every cell at t = 0
+  std::vector<T> rho0, u0;
+  {
+    auto& b = L.getBlock(0);
+    for (std::size_t c = 0; c < b.getNcells(); ++c) {
+      T rr, uu[3];
+      b.get(c).computeRhoU(rr, uu);
+      rho0.push_back(rr);
+      u0.insert(u0.end(), uu, uu + 3);
+    }
+  }
+  util::ValueTracer<T> converge(converter.getLatticeTime(physInterval), residuum);
+  std::vector<double> energy;
+  int iT;
+  for (iT = 0; iT < (int)converter.getLatticeTime(maxPhysT); ++iT) {
+    if (converge.hasConverged()) {
+      break;
+    }
+    L.collideAndStream();
+    energy.push_back(L.getStatistics().getAverageEnergy());
+    converge.takeValue(L.getStatistics().getAverageEnergy(), false);
+  }
+
+  int tmp[] = {int()};
+  T result[2] = {T(), T()}, resultSol[2] = {T(), T()};
+  const T maxVelocity = converter.getCharPhysVelocity();
+  CirclePoiseuille3D<T> uSol(axisPoint, axisDirection, maxVelocity, r);
+  SuperLatticePhysVelocity3D<T,DESCRIPTOR> u(L, converter);
+  SuperLatticeFfromAnalyticalF3D<T,DESCRIPTOR> uSolLattice(uSol, L);
+  SuperL2Norm3D<T> uL2Norm(uSolLattice - u, G, 1);
+  SuperL2Norm3D<T> uSolL2Norm(uSolLattice, G, 1);
+  uL2Norm(result, tmp);
+  uSolL2Norm(resultSol, tmp);
+  const double velocityErrorL2 = result[0] / resultSol[0];
+  double pressureErrorL2 = 0;
+  const T p0phys = 4. * converter.getPhysViscosity() * maxVelocity * lx / (r * r);
+  AnalyticalLinear3D<T,T> pressureSol(-p0phys / lx, 0, 0, p0phys);
+  if (!forced) {
+    SuperLatticePhysPressure3D<T,DESCRIPTOR> pressure(L, converter);
+    SuperLatticeFfromAnalyticalF3D<T,DESCRIPTOR> pressureSolLattice(pressureSol, L);
+    SuperL2Norm3D<T> pL2Norm(pressureSolLattice - pressure, G, 1);
+    SuperL2Norm3D<T> pSolL2Norm(pressureSolLattice, G, 1);
+    pL2Norm(result, tmp);
+    pSolL2Norm(resultSol, tmp);
+    pressureErrorL2 = result[0] / resultSol[0];
+  }
+
+  auto& bg = G.getBlockGeometry(0);
+  std::vector<T> usol, psol;
+  for (int x = -padding; x < bg.getNx() + padding; ++x) {
+    for (int y = -padding; y < bg.getNy() + padding; ++y) {
+      for (int z = -padding; z < bg.getNz() + padding; ++z) {
+        T v[3] = {0, 0, 0}, pv[1] = {0};
+        if (bg.get({x, y, z}) == 1) {
+          Vector<T,3> physR;
+          bg.getPhysR(physR, {x, y, z});
+          uSol(v, physR.data());
+          pressureSol(pv, physR.data());
+        }
+        usol.insert(usol.end(), v, v + 3);
+        psol.push_back(pv[0]);
+      }
+    }
+  }
+  char buf[768];
+  std::snprintf(buf, sizeof(buf),
+                " case=poiseuille boundary=local forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
+                "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu",
+                forced ? 1 : 0, n, iT, (double)omega, (double)force0,
+                (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
+                converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
+                converter.getLatticeTime(maxPhysT));
+  Dump d(out, block_header(L) + buf);
+  d.array("materials", "i4", materials(G, padding));
+  dump_normals(d, G, padding, {2, 3, 4});
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), f0);
+  d.array("f1", tname(), populations(L));
+  d.array("rho0", tname(), rho0);
+  d.array("u0", tname(), u0);
+  d.array("energy", "f8", energy);
+  d.array("usol", tname(), usol);
+  d.array("psol", tname(), psol);
+  return 0;
+}
+
 int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
@@ -402,11 +655,13 @@ int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
   L.template defineDynamics<NoDynamics>(G, 0);
   L.template defineDynamics<ForcedBGKdynamics>(G, 1);
   if (bouzidi) {
-    // the bouzidi row of the test (testPoiseuille3DBGK.cpp:141-151, forced)
+    // the bouzidi row of the test (testPoiseuille3DBGK.cpp:130-151, forced)
     L.template defineDynamics<NoDynamics>(G, 2);
     Vector<T,3> b0(0, r, r), b1(lx, r, r);
-    b0[0] -= 0.5 * dx + 3. * dx;
-    b1[0] += 0.5 * dx + 3. * dx;
+    b0[0] -= 0.5 * dx;
+    b1[0] += 0.5 * dx;
+    b0[0] -= 3. * dx;
+    b1[0] += 3. * dx;
     IndicatorCylinder3D<T> wall(b0, b1, r);
     setBouzidiBoundary<T, DESCRIPTOR, BouzidiPostProcessor>(L, G, 2, wall);
   } else {
@@ -472,14 +727,18 @@ int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
       }
     }
   }
-  char buf[512];
+  const auto& cuboid = G.getCuboidDecomposition().get(0);
+  const Vector<T,3> origin = cuboid.getOrigin();
+  const T delta = cuboid.getDeltaR();
+  char buf[768];
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g deltaT=%zu "
-                "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu",
+                "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu origin=%.17g,%.17g,%.17g delta=%.17g",
                 bouzidi ? "bouzidi" : "bounceBack", n, iT, (double)omega, (double)f[0],
                 (double)converter.getConversionFactorVelocity(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2,
-                converter.getLatticeTime(maxPhysT));
+                converter.getLatticeTime(maxPhysT), (double)origin[0], (double)origin[1],
+                (double)origin[2], (double)delta);
   Dump d(out, block_header(L) + buf);
   d.array("materials", "i4", materials(G, padding));
   d.array("distances", "i8", distances(L));
@@ -512,9 +771,12 @@ int main(int argc, char** argv)
   }
   const std::string mode = argv[1];
   try {
-    if ((mode == "cavity" || mode == "cavity_lv") && argc == 7) {
+    if ((mode == "cavity" || mode == "cavity_lv") && (argc == 7 || argc == 8)) {
       return run_cavity(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6],
-                        mode == "cavity_lv");
+                        mode == "cavity_lv", argc == 8 ? argv[7] : "");
+    }
+    if (mode == "vtk_tgv" && argc == 6) {
+      return run_vtk_tgv(std::stoi(argv[2]), std::stoi(argv[3]), argv[4], argv[5]);
     }
     if (mode == "tgv" && argc == 7) {
       return run_tgv(std::stoi(argv[2]), std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6], false);
@@ -534,6 +796,9 @@ int main(int argc, char** argv)
     }
     if (mode == "poiseuille" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3]);
+    }
+    if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && argc == 4) {
+      return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local");
     }
     if (mode == "poiseuille_bouzidi" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3], true);

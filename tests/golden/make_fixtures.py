@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import json
 import os
+import shutil
 import subprocess
 import sys
 
@@ -46,7 +47,7 @@ def core_ids(meta) -> np.ndarray:
     return idx[p:-p, p:-p, p:-p].reshape(-1)
 
 
-def save(name: str, meta: dict, arrays: dict, extra: dict | None = None, cells=None) -> None:
+def save(name: str, meta: dict, arrays: dict, extra: dict | None = None, cells=None, texts=None) -> None:
     if cells is None:
         cells = core_ids(meta)
     q = 19
@@ -65,6 +66,8 @@ def save(name: str, meta: dict, arrays: dict, extra: dict | None = None, cells=N
     if "normal_type" in arrays:
         out["normal_type"] = np.asarray(arrays["normal_type"]).astype(np.int8)
         out["normals"] = np.asarray(arrays["normals"]).astype(np.int8)
+    for k, v in (texts or {}).items():
+        out[k] = np.array(v)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print("wrote", name, {k: v.shape for k, v in out.items() if hasattr(v, "shape")})
 
@@ -226,6 +229,10 @@ def main() -> None:
                 smagorinsky_fixtures()
             elif name == "checkpoint":
                 checkpoint_fixtures()
+            elif name == "bouzidi":
+                poiseuille_bouzidi()
+            elif name == "vtk":
+                vtk_fixtures()
             else:
                 full_size(name)
         return
@@ -272,6 +279,77 @@ def main() -> None:
     }
     np.savez_compressed(os.path.join(HERE, "poiseuille11_forced_bounceback_f64.npz"), **out)
     print("wrote poiseuille", meta)
+    poiseuille_bouzidi()
+
+
+def vtk_fixtures() -> None:
+    """SuperVTMwriter3D output of the reference (velocity, density,
+    physVelocity, physPressure; compressed and plain binary) for a periodic
+    Taylor-Green vortex 6^3 after 10 steps (converter factors 2 and 4.8) and
+    the bounce-back and LocalVelocity cavities at RESOLUTION 8 after 20
+    steps, FP32, with the populations they were written from"""
+    runs = [("vtk_tgv6", "tgv", ["vtk_tgv", "6", "10"]),
+            ("vtk_cavity8", "cavity", ["cavity", "8", "20", "0.1", "100"]),
+            ("vtk_cavity_lv8", "cavity_lv", ["cavity_lv", "8", "20", "0.1", "100"])]
+    for fixture, name, args in runs:
+        path = os.path.join(RUN, fixture + ".bin")
+        out_dir = os.path.join(RUN, fixture)
+        shutil.rmtree(out_dir, ignore_errors=True)
+        run("olb_ref_f", [*args, path, out_dir])
+        meta, arrays = read_dump(path)
+        steps = int(args[2])
+        vtk = os.path.join(out_dir, "vtkData")
+        texts = {}
+        for key, stem in (("vti", name), ("vti_raw", name + "_raw")):
+            with open(os.path.join(vtk, "data", f"{stem}_iT{steps:07d}iC00000.vti")) as fh:
+                texts[key] = fh.read()
+        with open(os.path.join(vtk, "data", f"{name}_iT{steps:07d}.vtm")) as fh:
+            texts["vtm"] = fh.read()
+        with open(os.path.join(vtk, f"{name}.pvd")) as fh:
+            texts["pvd"] = fh.read()
+        save(fixture, meta, arrays, extra={"vtk_name": name}, texts=texts)
+
+
+def poiseuille_bouzidi() -> None:
+    """testPoiseuille3DBGK's forced Bouzidi row at N = 11, FP64, run to
+    convergence by the reference (one thread: the reference applies its
+    post-processors in ascending cell id only when OpenMP runs one thread)"""
+    from openlb_amd import bouzidi
+
+    path = os.path.join(RUN, "poiseuille_bouzidi.bin")
+    run("olb_ref_d", ["poiseuille_bouzidi", "11", path], threads=1)
+    meta, arrays = read_dump(path)
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    mats = arrays["materials"]
+    keep = np.flatnonzero(mats != 0)
+    fluid = np.flatnonzero(mats == 1)
+    q = 19
+    n = arrays["f0"].size // q
+    field = np.asarray(arrays["bouzidi"]).reshape(q, n)
+    # cells that run the post-processor: every non-padding cell with an entry
+    core = np.zeros(shape, dtype=bool)
+    core[p:-p, p:-p, p:-p] = True
+    bz_cells = np.flatnonzero((field >= 0).any(axis=0) & core.reshape(-1))
+    entries = np.flatnonzero((field >= 0).any(axis=0))
+    out = {
+        "meta": np.array(json.dumps(meta)),
+        "materials": mats.astype(np.uint8),
+        "distances": arrays["distances"],
+        "cells": keep.astype(np.int64),
+        "f0": arrays["f0"].reshape(q, n)[:, keep],
+        "fluid": fluid.astype(np.int64),
+        "f1_fluid": arrays["f1"].reshape(q, n)[:, fluid],
+        "f1_bz": arrays["f1"].reshape(q, n)[:, bz_cells],
+        "energy": arrays["energy"],
+        "usol_fluid": arrays["usol"].reshape(n, 3)[fluid],
+        "bz_cells": bz_cells.astype(np.int64),
+        "bz_q": np.ascontiguousarray(field[:, bz_cells].T),
+        "field_cells": entries.astype(np.int64),
+        "field_q": np.ascontiguousarray(field[:, entries].T),
+    }
+    np.savez_compressed(os.path.join(HERE, "poiseuille11_forced_bouzidi_f64.npz"), **out)
+    print("wrote poiseuille bouzidi", meta)
 
 
 if __name__ == "__main__":

@@ -56,8 +56,8 @@ def dynamics_for(meta: dict, storage: str) -> dict:
                             force=(_as_value(meta["fx"], storage), 0.0, 0.0)),
                 2: Dynamics(BOUNCE_BACK)}
     if case == "poiseuille":
-        return {1: Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0)),
-                2: Dynamics(BOUNCE_BACK)}
+        wall = Dynamics(NO_DYNAMICS) if meta.get("boundary") == "bouzidi" else Dynamics(BOUNCE_BACK)
+        return {1: Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0)), 2: wall}
     raise ValueError(case)
 
 
@@ -98,10 +98,45 @@ def build(lattice_cls, meta, d, storage: str):
             mask[on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))] = fixed
         lat.set_cell_normals(d["normals"])
     lat.set_dynamics_mask(mask)
+    if "field_cells" in d:
+        # the reference's own BOUZIDI_DISTANCE field, solid-solid pairs resolved
+        lat.set_bouzidi(*bouzidi_entries(meta, d))
     f = np.zeros((19, lat.ncells))
     f[:, d["cells"]] = d["f0"]
     lat.set_populations(f)
     return lat
+
+
+def padded_shape(meta) -> tuple:
+    p = meta["padding"]
+    return (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+
+
+def bouzidi_field(meta, d) -> np.ndarray:
+    """the reference's BOUZIDI_DISTANCE field, 19 x padded cells (-1 unset)"""
+    field = np.full((19, int(np.prod(padded_shape(meta)))), -1.0)
+    field[:, d["field_cells"]] = d["field_q"].T
+    return field
+
+
+def bouzidi_entries(meta, d):
+    from openlb_amd import bouzidi
+
+    return bouzidi.entries(bouzidi_field(meta, d), meta["padding"], padded_shape(meta))
+
+
+def poiseuille_wall(meta):
+    """the Bouzidi wall indicator of testPoiseuille3DBGK.cpp:130-151 (forced):
+    the pipe cylinder stretched by dx/2 + 3 dx at both ends"""
+    from openlb_amd import bouzidi
+
+    dx, r, lx = meta["delta"], 0.5, 2.0
+    c0, c1 = [0.0, r, r], [lx, r, r]
+    c0[0] -= 0.5 * dx
+    c1[0] += 0.5 * dx
+    c0[0] -= 3.0 * dx
+    c1[0] += 3.0 * dx
+    return bouzidi.IndicatorCylinder3D(c0, c1, r)
 
 
 def compared_cells(meta, d) -> np.ndarray:

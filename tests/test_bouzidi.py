@@ -1,0 +1,138 @@
+"""Bouzidi interpolated bounce-back on the CPU: the setter's distance field
+against the reference's own, the ray distance against the closed-form
+ray-cylinder intersection, the concurrent entry list against the reference's
+sequential order, and the oracle against the reference's known-answer test
+(testPoiseuille3DBGK.cpp:57, forced, bouzidi, N = 11: iT = 294, velocity L2
+error 0.019703 +- 1e-6)."""
+import math
+
+import numpy as np
+import pytest
+
+from openlb_amd import bouzidi
+from oracle.oracle import OracleLattice
+from tests.golden_runs import (bouzidi_entries, bouzidi_field, build, core_fluid, load, padded_shape,
+                               poiseuille_wall, value_tracer_run, velocity_error_l2)
+from tests.helpers import rel_err_unshifted
+
+NAME = "poiseuille11_forced_bouzidi_f64"
+
+
+def _origin(meta):
+    return [float(v) for v in str(meta["origin"]).split(",")]
+
+
+def test_distance_field_matches_reference():
+    """every BOUZIDI_DISTANCE entry the reference's setter wrote, from the
+    geometry and the wall cylinder alone: same set of entries, values equal
+    to the last bit or two (the reference contracts a*b+c into FMAs)"""
+    meta, d = load(NAME)
+    mats = d["materials"].reshape(padded_shape(meta))
+    q = bouzidi.distances(mats, meta["padding"], _origin(meta), meta["delta"], poiseuille_wall(meta), 2)
+    ref = bouzidi_field(meta, d)
+    np.testing.assert_array_equal(q >= 0, ref >= 0)
+    assert np.max(np.abs(q - ref)) <= 1e-15
+    # the fall-backs and the interpolated entries are both present
+    on = ref >= 0
+    assert np.sum(ref[on] == 0.5) > 0 and np.sum((ref[on] > 0) & (ref[on] != 0.5)) > 0
+
+
+def _ray_cylinder(p, v, c, r):
+    """closed form: smallest t > 0 with |(p + t v - c)_yz| = r for a cylinder
+    along x through c (yz components only)"""
+    py, pz = p[1] - c[1], p[2] - c[2]
+    vy, vz = v[1], v[2]
+    a = vy * vy + vz * vz
+    b = 2 * (py * vy + pz * vz)
+    cc = py * py + pz * pz - r * r
+    disc = b * b - 4 * a * cc
+    if a == 0 or disc < 0:
+        return None
+    ts = sorted(((-b - math.sqrt(disc)) / (2 * a), (-b + math.sqrt(disc)) / (2 * a)))
+    ts = [t for t in ts if t > 0]
+    return ts[0] if ts else None
+
+
+def test_ray_distance_against_closed_form():
+    """the bisection of util::distance (precision 1e-4, pitch 1/2 in units of
+    the direction) finds the ray-cylinder intersection of the mathematics to
+    within its precision, from points inside the cylinder (where the setter
+    starts: the fluid cells) up to two lattice links from the wall"""
+    rng = np.random.default_rng(7)
+    cyl = bouzidi.IndicatorCylinder3D([-1.0, 0.5, 0.5], [3.0, 0.5, 0.5], 0.5)
+    dx = 0.05
+    checked = 0
+    for _ in range(3000):
+        p = [rng.uniform(0, 2), *rng.uniform(0.0, 1.0, 2)]
+        i = int(rng.integers(1, 19))
+        v = [-dx * float(c) for c in bouzidi.C[i]]
+        if v[1] == 0 and v[2] == 0:
+            continue
+        if not cyl.inside(p):
+            continue
+        exact = _ray_cylinder(p, v, [0, 0.5, 0.5], 0.5)
+        got = bouzidi.ray_distance(cyl, p, v)
+        if exact is None or exact > 2:
+            continue
+        norm = math.sqrt(sum(c * c for c in v))
+        assert got is not None
+        assert abs(got / norm - exact) <= 2e-4, (p, i, got / norm, exact)
+        checked += 1
+    assert checked > 100
+
+
+def test_entries_reproduce_the_sequential_order():
+    """the concurrent entry list (solid-solid q = 1/2 pairs resolved in
+    ascending cell id) gives, applied one cell after another, exactly what the
+    reference's full list gives in its ascending order, on every cell"""
+    meta, d = load(NAME)
+    cells, q = bouzidi_entries(meta, d)
+    full_cells, full_q = d["bz_cells"], d["bz_q"]
+    assert cells.size == full_cells.size and (q >= 0).sum() < (full_q >= 0).sum()
+    runs = []
+    for cl, qq in ((full_cells, full_q), (cells, q)):
+        lat = build(OracleLattice, meta, d, "f64")
+        lat.set_bouzidi(cl, qq)
+        lat.collide_and_stream(60)
+        runs.append(lat.get_populations())
+    np.testing.assert_array_equal(runs[0], runs[1])
+
+
+def test_entries_reject_mutual_reads():
+    """two fluid cells whose interpolations read each other's outputs cannot
+    be updated concurrently"""
+    shape = (6, 6, 6)
+    q = np.full((19, 216), -1.0)
+    a = np.ravel_multi_index((2, 2, 2), shape)
+    b = np.ravel_multi_index((3, 2, 2), shape)  # a + c_10
+    q[10, a] = 0.3  # a writes a[1], reads b[10]
+    q[1, b] = 0.3   # b writes b[10], reads a[1]
+    with pytest.raises(ValueError):
+        bouzidi.entries(q, 1, shape)
+
+
+def test_oracle_poiseuille_bouzidi_known_answer():
+    meta, d = load(NAME)
+    lat = build(OracleLattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 294 == meta["iT"]
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    print("oracle bouzidi velocity error", err)
+    assert abs(err - 0.019703) <= 1e-6
+    assert abs(err - meta["velocityErrorL2"]) <= 1e-9
+    sel = core_fluid(meta, d)
+    f1 = lat.get_populations()[:, d["fluid"][sel]]
+    assert rel_err_unshifted(f1, d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_oracle_poiseuille_bouzidi_with_own_distances():
+    """the same run from the distances this package computes"""
+    meta, d = load(NAME)
+    lat = build(OracleLattice, meta, d, "f64")
+    mats = d["materials"].reshape(padded_shape(meta))
+    q = bouzidi.distances(mats, meta["padding"], _origin(meta), meta["delta"], poiseuille_wall(meta), 2)
+    lat.set_bouzidi(*bouzidi.entries(q, meta["padding"], padded_shape(meta)))
+    it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 294
+    assert abs(velocity_error_l2(lat, meta, d) - 0.019703) <= 1e-6

@@ -142,3 +142,124 @@ def test_gpu_local_velocity_cavity_res64_matches_reference(storage, tol_u):
     if storage == "f32":
         assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5
         np.testing.assert_allclose(sums[4], d["fluid_sums"][4], rtol=1e-4)
+
+
+def test_gpu_poiseuille_bouzidi_known_answer():
+    """testPoiseuille3DBGK.cpp:57 (forced, bouzidi, N = 11) on the GPU in FP64
+    from the reference's own distance field: iteration count 294, velocity L2
+    error 0.019703 +- 1e-6, fluid populations within 1e-12 of the reference"""
+    meta, d = load("poiseuille11_forced_bouzidi_f64")
+    lat = build(Lattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 294
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    print("poiseuille bouzidi gpu velocity error", err)
+    assert abs(err - 0.019703) <= 1e-6
+    assert abs(err - meta["velocityErrorL2"]) <= 1e-9
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_gpu_bouzidi_matches_oracle_on_every_core_cell(storage):
+    """the concurrent Bouzidi kernel against the oracle applying the
+    reference's full entry list one cell after another (its ascending cell
+    id), after 60 steps, on every population of every core cell -- fluid and
+    solid -- whose value does not depend on what the padding holds (solid
+    and outside cells at the non-periodic faces pull values from the padding,
+    which neither the reference nor this platform defines; two more oracle
+    runs whose padding is refilled with different random values before every
+    step mark them); FP32 within 1e-5, FP64 within 1e-12 relative"""
+    from oracle.oracle import OracleLattice
+    from tests.golden_runs import padded_shape
+
+    meta, d = load("poiseuille11_forced_bouzidi_f64")
+    p = meta["padding"]
+    core = np.zeros(padded_shape(meta), dtype=bool)
+    core[p:-p, p:-p, p:-p] = True
+    runs = []
+    pad = ~core.reshape(-1)
+    for seed in (None, 1, 2):
+        ora = build(OracleLattice, meta, d, storage)
+        ora.set_bouzidi(d["bz_cells"], d["bz_q"])
+        if seed is None:
+            ora.collide_and_stream(60)
+        else:
+            rng = np.random.default_rng(seed)
+            for _ in range(60):
+                f = ora.get_populations()
+                f[:, pad] = rng.uniform(-0.01, 0.01, size=(19, int(pad.sum())))
+                ora.set_populations(f)
+                ora.collide_and_stream(1)
+        runs.append(ora.get_populations())
+    lat = build(Lattice, meta, d, storage)
+    lat.collide_and_stream(60)
+    core = np.flatnonzero(core)
+    defined = runs[1][:, core] == runs[2][:, core]
+    mats = d["materials"][core]
+    # the fluid and every Bouzidi population the fluid reads are defined
+    assert defined[:, mats == 1].all()
+    listed = np.isin(core, d["bz_cells"])
+    print(storage, "defined populations of listed solid cells", float(defined[:, listed & (mats == 2)].mean()))
+    got, ref = lat.get_populations()[:, core], runs[0][:, core]
+    from tests.helpers import T as W
+
+    rel = np.where(defined, np.abs(got.astype(np.float64) - ref) / np.abs(ref + W[:, None]), 0.0)
+    err = float(rel.max())
+    print(storage, "bouzidi gpu vs oracle", err)
+    if err > TOL[storage]:
+        for m in np.unique(mats):
+            print("material", m, "max rel", float(rel[:, mats == m].max()))
+        worst = np.argsort(rel.max(axis=0))[::-1][:8]
+        for c in worst:
+            xyz = np.unravel_index(core[c], padded_shape(meta))
+            i = int(np.argmax(rel[:, c]))
+            print("cell", [int(v) - p for v in xyz], "material", int(mats[c]), "pop", i,
+                  "gpu", float(got[i, c]), "oracle", float(ref[i, c]), "listed", bool(listed[c]))
+    assert err <= TOL[storage]
+
+
+def test_gpu_bouzidi_fp16_storage_near_fp64():
+    """FP16 shifted storage with Bouzidi walls: fluid velocity within 1e-2
+    relative L2 of the FP64 run after 300 steps"""
+    from tests.helpers import rel_l2, velocity
+
+    meta, d = load("poiseuille11_forced_bouzidi_f64")
+    sel = d["fluid"][core_fluid(meta, d)]
+    us = {}
+    for storage in ("f16", "f64"):
+        lat = build(Lattice, meta, d, storage)
+        lat.collide_and_stream(300)
+        us[storage] = velocity(lat.get_populations()[:, sel].astype(np.float64))
+    err = rel_l2(us["f16"], us["f64"])
+    print("bouzidi fp16 vs fp64 velocity rel L2", err)
+    assert err <= 1e-2
+
+
+def test_gpu_bouzidi_rejects_bad_lists():
+    from openlb_amd import Ensemble, OlbgfxError
+
+    L = Lattice((6, 6, 6), padding=1)
+    q = np.full((1, 19), -1.0)
+    q[0, 10] = 0.3
+    with pytest.raises(OlbgfxError):  # a padding cell
+        L.set_bouzidi([0], q)
+    a, b = L.cell_id(2, 2, 2), L.cell_id(3, 2, 2)
+    with pytest.raises(OlbgfxError):  # not ascending
+        L.set_bouzidi([b, a], np.vstack([q, q]))
+    bad = q.copy()
+    bad[0, 3] = np.nan
+    with pytest.raises(OlbgfxError):
+        L.set_bouzidi([a], bad)
+    # a writes a[1] reading b[10]; b writes b[10] reading a[1]
+    qb = np.full((1, 19), -1.0)
+    qb[0, 1] = 0.3
+    with pytest.raises(OlbgfxError):
+        L.set_bouzidi([a, b], np.vstack([q, qb]))
+    L.set_bouzidi([a], q)
+    with pytest.raises(OlbgfxError) as e:
+        Ensemble([L]).step(1)
+    assert e.value.status == 2
+    L.set_bouzidi([], np.zeros((0, 19)))
+    L.close()

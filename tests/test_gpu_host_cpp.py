@@ -71,3 +71,44 @@ def test_cpp_checkpoint_save_and_restart(tmp_path):
     f_straight = run(20)                       # 10 warm-up + 20
     core = c.materials.reshape(-1) != 0
     np.testing.assert_array_equal(f_restart[:, core], f_straight[:, core])
+
+
+@pytest.mark.parametrize("boundary,fixture", [("bounce_back", "vtk_cavity8"), ("local_velocity", "vtk_cavity_lv8")])
+def test_cpp_cavity_vtk_matches_reference(tmp_path, boundary, fixture):
+    """the C++ mirror's SuperVTMwriter3D after 20 steps of the RESOLUTION 8
+    cavity against the reference's own VTK output of the same run: every
+    point value of every functor (FP32), the .vtm and the .pvd text"""
+    from openlb_amd import vtk
+    from tests.golden_runs import load
+
+    meta, d = load(fixture)
+    print("[cpp] cavity3d vtk", boundary, flush=True)
+    dump = tmp_path / "cavity.bin"
+    out = subprocess.run([BINARY, "--RESOLUTION", "8", "--TIME_STEPS", "10", "--BOUNDARY", boundary,
+                          "--VTK", str(tmp_path), "--DUMP", str(dump)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    # the run itself against the reference's populations after 20 steps (fluid)
+    raw = np.fromfile(dump, dtype=np.uint8)
+    ncells = d["materials"].size
+    np.testing.assert_array_equal(raw[:ncells], d["materials"])
+    f = raw[ncells:].view(np.float32).reshape(19, ncells)
+    fluid = d["materials"][d["cells"]] == 1
+    assert rel_err_unshifted(f[:, d["cells"]][:, fluid], d["f1"][:, fluid]) <= 1e-5
+    name = meta["vtk_name"]
+    ours = vtk.read_vti(str(tmp_path / "data" / (vtk.file_name(name, 20, 0) + ".vti")))
+    ref = vtk.read_vti_text(str(d["vti"]))
+    for fn in vtk.FUNCTORS:
+        e = np.abs(ours[fn].astype(np.float64) - ref[fn])
+        err = float(e.max())
+        print(boundary, fn, "max abs err", err)
+        if err > 2e-6:
+            k = int(np.argmax(e)) // vtk.NCOMP[fn]
+            nx, ny = meta["nx"] + 2, meta["ny"] + 2
+            x, y, z = k % nx - 1, (k // nx) % ny - 1, k // (nx * ny) - 1
+            print("worst point", (x, y, z), "ours", ours[fn][k * vtk.NCOMP[fn]:(k + 1) * vtk.NCOMP[fn]],
+                  "reference", ref[fn][k * vtk.NCOMP[fn]:(k + 1) * vtk.NCOMP[fn]])
+        assert err <= 2e-6
+    assert (tmp_path / "data" / (vtk.file_name(name, 20) + ".vtm")).read_text() == str(d["vtm"])
+    assert (tmp_path / (name + ".pvd")).read_text() == str(d["pvd"])
+    head = lambda t: [ln for ln in t.splitlines() if ln.startswith("<")]  # noqa: E731
+    assert head((tmp_path / "data" / (vtk.file_name(name, 20, 0) + ".vti")).read_text()) == head(str(d["vti"]))

@@ -41,3 +41,29 @@ def test_cpp_base64_reproduces_reference_checkpoint(tmp_path):
                          capture_output=True, text=True, check=True).stdout.split()
     assert out[0] == out[1] == str(19 * (16 + 2 * 1024 * 4))
     assert out[2] == "same"
+
+
+def test_cpp_bouzidi_distances_match_reference(tmp_path):
+    """the mirror's setBouzidiBoundary field (IndicatorCylinder3D and the
+    bisection ray distance in C++) equals the field the reference's setter
+    wrote for its forced Bouzidi Poiseuille pipe, entry for entry"""
+    from tests.golden_runs import bouzidi_field
+
+    exe = tmp_path / "bouzidi_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "openlb_amd", "host"),
+                    os.path.join(ROOT, "tests", "cpp", "bouzidi_check.cpp"), "-o", str(exe)], check=True)
+    meta, d = load("poiseuille11_forced_bouzidi_f64")
+    mats = tmp_path / "materials.bin"
+    d["materials"].astype(np.uint8).tofile(mats)
+    origin = str(meta["origin"]).split(",")
+    out = subprocess.run([str(exe), str(mats), str(meta["nx"]), str(meta["ny"]), str(meta["nz"]),
+                          str(meta["padding"]), *origin, repr(meta["delta"])],
+                         capture_output=True, text=True, check=True).stdout
+    ref = bouzidi_field(meta, d)
+    got = np.full_like(ref, -1.0)
+    for line in out.strip().splitlines():
+        c, i, q = line.split()
+        got[int(i), int(c)] = float(q)
+    np.testing.assert_array_equal(got >= 0, ref >= 0)
+    assert np.max(np.abs(got - ref)) <= 1e-15
