@@ -68,6 +68,22 @@ typedef int32_t olbgfx_status;
  * the effective relaxation of collision::SmagorinskyEffectiveOmega
  * (dynamics/collisionLES.h:59-89), constant `smagorinsky` */
 #define OLBGFX_SMAGORINSKY_BGK      9
+/* LocalVelocity on other boundary shapes (boundary/localVelocity3D.h:86-99)
+ * and LocalPressure faces (boundary/localPressure3D.h:47-58), all
+ * CombinedRLBdynamics around BGK that need the cell's discrete normal:
+ *   INNER_EDGE_VELOCITY: InnerEdgeVelocityTuple3D (normal with two non-zero
+ *     entries): density the mean of velocityBMRho over both walls, stress
+ *     with the populations entering from both walls bounced back
+ *     (momenta/elements.h:459-482, 1871-1892);
+ *   INNER_CORNER_VELOCITY: InnerCornerVelocityTuple3D (three non-zero
+ *     entries; elements.h:432-457, 1843-1869);
+ *   LOCAL_PRESSURE: RegularizedPressureBoundaryTuple (flat normal): density
+ *     fixed at wall_rho or the RHO field, velocity the wall's tangential
+ *     VELOCITY with the normal component from the known populations
+ *     (momenta/aliases.h:152-159, elements.h:653-691). */
+#define OLBGFX_INNER_EDGE_VELOCITY   10
+#define OLBGFX_INNER_CORNER_VELOCITY 11
+#define OLBGFX_LOCAL_PRESSURE        12
 
 /* per-cell fields */
 #define OLBGFX_FIELD_VELOCITY 0  /* descriptors::VELOCITY, 3 values per cell */
@@ -174,6 +190,13 @@ olbgfx_status olbgfx_set_populations(olbgfx_lattice* lattice, int32_t value_type
  * collide every core cell, fill the periodic overlap, rotate the rings.
  * Enqueued on `stream`; does not synchronise. */
 olbgfx_status olbgfx_collide_and_stream(olbgfx_lattice* lattice, uint32_t steps, void* stream);
+
+/* BlockLattice::postProcess() of the PostStream stage on the current
+ * populations, without a step: the Bouzidi, LocalVelocity edge and corner
+ * processors.  BlockLattice::initialize (core/blockLattice.hh:56-67) runs it
+ * once before the first step, so a case that starts from its own initial
+ * values calls it once after setting them.  Asynchronous on `stream`. */
+olbgfx_status olbgfx_post_process(olbgfx_lattice* lattice, void* stream);
 
 /* ---- statistics and reductions -------------------------------------------
  * BlockLattice::setStatisticsEnabled (core/blockLattice.h:239-244).  With statistics on,

@@ -31,6 +31,7 @@ STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
 VALUE_F32, VALUE_F64 = 0, 1
 NO_DYNAMICS, BGK, TRT, FORCED_BGK, FORCED_TRT, BOUNCE_BACK, BOUNCE_BACK_VELOCITY = range(7)
 LOCAL_VELOCITY, FIXED_VELOCITY, SMAGORINSKY_BGK = 7, 8, 9
+INNER_EDGE_VELOCITY, INNER_CORNER_VELOCITY, LOCAL_PRESSURE = 10, 11, 12
 FIELD_VELOCITY, FIELD_RHO, FIELD_FORCE = 0, 1, 2
 MAX_DYNAMICS = 16
 Q = 19
@@ -103,6 +104,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_set_cell_field": [vp, i32, i32, vp],
         "olbgfx_set_cell_normals": [vp, vp],
         "olbgfx_set_bouzidi": [vp, ctypes.c_int64, vp, i32, vp],
+        "olbgfx_post_process": [vp, vp],
         "olbgfx_ini_equilibrium": [vp, vp, i32, vp, vp],
         "olbgfx_get_populations": [vp, i32, vp],
         "olbgfx_set_populations": [vp, i32, vp],
@@ -282,6 +284,12 @@ class Lattice:
         if v.size != 3 * self.ncells:
             raise ValueError("normals must hold 3 entries per cell")
         _check(library().olbgfx_set_cell_normals(self._h, _ptr(v)))
+
+    def post_process(self, stream: int | None = None) -> None:
+        """BlockLattice::postProcess(): the PostStream processors (Bouzidi,
+        LocalVelocity edges and corners) on the current populations; the
+        reference runs it once in initialize()"""
+        _check(library().olbgfx_post_process(self._h, stream))
 
     def set_bouzidi(self, cells, q) -> None:
         """setBouzidiBoundary: the core cells (ascending ids) that run

@@ -57,11 +57,12 @@ def build_library() -> str:
 def build_host_examples() -> None:
     """C++ cases written against the host mirror (openlb_amd/host); plain g++
     over the C ABI, no HIP headers"""
-    src = os.path.join(PKG, "host", "examples", "cavity3d.cpp")
-    out = os.path.join(PKG, "host", "examples", "cavity3d")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-          "-I", os.path.join(PKG, "host"), src, "-L", PKG, "-lolbgfx",
-          "-Wl,-rpath,$ORIGIN/../..", "-lz", "-o", out])
+    for name in ("cavity3d", "channel3d"):
+        src = os.path.join(PKG, "host", "examples", name + ".cpp")
+        out = os.path.join(PKG, "host", "examples", name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+              "-I", os.path.join(PKG, "host"), src, "-L", PKG, "-lolbgfx",
+              "-Wl,-rpath,$ORIGIN/../..", "-lz", "-o", out])
 
 
 def build_oracle(with_reference: bool = True) -> None:

@@ -17,7 +17,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT, LOCAL_VELOCITY,
+from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT, LOCAL_PRESSURE,
+               LOCAL_VELOCITY,
                NO_DYNAMICS, Dynamics)
 
 __all__ = [
@@ -75,13 +76,17 @@ class Case:
                 slots[(mat, "flat")] = len(table) - 1
                 table.append(Dynamics(FIXED_VELOCITY, omega=d.omega, wall_velocity=d.wall_velocity))
                 slots[(mat, "fixed")] = len(table) - 1
+            elif isinstance(d, LocalPressure):
+                table.append(Dynamics(LOCAL_PRESSURE, omega=d.omega, wall_rho=d.rho))
+                slots[(mat, "flat")] = len(table) - 1
             else:
                 table.append(d)
                 slots[mat] = len(table) - 1
         return slots, table
 
     def local_velocity_materials(self) -> tuple:
-        return tuple(m for m, d in sorted(self.dynamics.items()) if isinstance(d, LocalVelocity))
+        """the materials with local boundaries (LocalVelocity or LocalPressure)"""
+        return tuple(m for m, d in sorted(self.dynamics.items()) if isinstance(d, (LocalVelocity, LocalPressure)))
 
     def cell_normals(self):
         """(type, normal) per cell for the LocalVelocity materials, or None"""
@@ -106,9 +111,14 @@ class Case:
             for m in self.local_velocity_materials():
                 on = mats == m
                 flat = on & (types == FLAT)
+                if isinstance(self.dynamics[m], LocalPressure):
+                    if np.any(on & ~flat):
+                        raise ValueError("LocalPressure takes flat faces only")
+                    mask[flat] = slots[(m, "flat")]
+                    continue
                 fixed = on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))
                 if np.any(on & ~flat & ~fixed):
-                    raise ValueError("LocalVelocity supports flat faces, external edges and external "
+                    raise ValueError("the box classifier covers flat faces, external edges and external "
                                      "corners; this geometry has other boundary cells")
                 mask[flat] = slots[(m, "flat")]
                 mask[fixed] = slots[(m, "fixed")]
@@ -127,6 +137,10 @@ class Case:
         if self.rho is not None:
             sel = np.isin(self.materials.reshape(-1), self.init_materials).astype(np.uint8)
             lattice.ini_equilibrium(self.rho.reshape(-1), self.u.reshape(-1, 3), sel)
+        if normals is not None:
+            # SuperLattice::initialize runs the PostStream processors once
+            # (core/blockLattice.hh:56-67)
+            lattice.post_process()
 
     def core_mask(self) -> np.ndarray:
         p = self.padding
@@ -146,6 +160,21 @@ class LocalVelocity:
 
     omega: float
     wall_velocity: tuple = (0.0, 0.0, 0.0)
+    # dynamics of the material's cells that get no boundary dynamics (no
+    # discrete normal; boundary::set leaves what defineDynamics gave them)
+    base: object = None
+
+
+@dataclass
+class LocalPressure:
+    """boundary::set<boundary::LocalPressure<T, D, BGKdynamics>> on a material
+    (boundary/localPressure3D.h:36-62): flat faces only, CombinedRLBdynamics
+    with RegularizedPressureBoundaryTuple; the density is the RHO the case
+    defines on the material."""
+
+    omega: float
+    rho: float = 1.0
+    base: object = None
 
 
 # DiscreteNormalType (geometry/discreteNormals.h:33-39)
@@ -459,3 +488,36 @@ def cavity3d(res: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,
     c.dynamics[1] = Dynamics(BGK, omega=omega)
     c.info.update({"nu": nu, "omega": omega, "res": res})
     return c
+
+
+def channel3d(nx: int, ny: int, nz: int, u_in: float = 0.05, tau: float = 0.8, padding: int = 3,
+              dtype=np.float64) -> Case:
+    """A straight channel along x on an nx x ny x nz core: BounceBack walls
+    (material 2) on the y and z faces, the x = 0 face without its rim a
+    LocalVelocity inflow at (u_in, 0, 0) (material 3), the x = nx-1 face
+    without its rim a LocalPressure outflow at rho = 1 (material 4), BGK fluid
+    (material 1) starting at rest."""
+    extent = (nx, ny, nz)
+    shape = _padded(extent, padding)
+    p = padding
+    mats = np.zeros(shape, dtype=np.uint8)
+    core = mats[p:-p, p:-p, p:-p]
+    core[...] = 2
+    core[:, 1:-1, 1:-1] = 1
+    core[0, 1:-1, 1:-1] = 3
+    core[-1, 1:-1, 1:-1] = 4
+    omega = 1.0 / tau
+    return Case(
+        name=f"channel{nx}x{ny}x{nz}",
+        extent=extent,
+        padding=padding,
+        periodic=(False, False, False),
+        materials=mats,
+        dynamics={1: Dynamics(BGK, omega=omega), 2: Dynamics(BOUNCE_BACK),
+                  3: LocalVelocity(omega=omega, wall_velocity=(u_in, 0.0, 0.0)),
+                  4: LocalPressure(omega=omega, rho=1.0)},
+        rho=np.ones(shape, dtype=dtype),
+        u=np.zeros(shape + (3,), dtype=dtype),
+        init_materials=(1, 2, 3, 4),
+        info={"omega": omega, "u_in": u_in},
+    )

@@ -47,9 +47,12 @@ constexpr int Q = 19;
 constexpr uint32_t kMaxBlock = 256;
 constexpr int kMaskedBoundary = -2;  // step-kernel MODE: masked, LocalVelocity kinds compiled in
 
+// the boundary kinds that read the cell's discrete normal and run in the
+// masked kernel with boundary kinds compiled in
 __host__ __device__ constexpr bool is_velocity_boundary(int kind)
 {
-  return kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_FIXED_VELOCITY;
+  return kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_FIXED_VELOCITY || kind == OLBGFX_INNER_EDGE_VELOCITY ||
+         kind == OLBGFX_INNER_CORNER_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE;
 }
 // LDS of one gfx950 compute unit
 constexpr uint32_t kLdsPerCu = 160 * 1024;
@@ -607,8 +610,156 @@ __device__ __forceinline__ void local_velocity(A (&f)[Q], int dir, int orient, c
   rho_out = rho;
 }
 
-// LOCAL_VELOCITY and FIXED_VELOCITY cells (the masked kernel with boundary
-// kinds); returns true: both yield a statistic
+// lbm<D>::computeStress (dynamics/lbm.h:334-352) of shifted populations
+template <typename A>
+__device__ __forceinline__ void bulk_stress(const A (&g)[Q], A rho, const A (&u)[3], A (&pi)[6])
+{
+  A s[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const A cx = A(CX(i)), cy = A(CY(i)), cz = A(CZ(i));
+    s[0] += cx * cx * g[i];
+    s[1] += cx * cy * g[i];
+    s[2] += cx * cz * g[i];
+    s[3] += cy * cy * g[i];
+    s[4] += cy * cz * g[i];
+    s[5] += cz * cz * g[i];
+  }
+  const A iso = A(1) / A(3) * (rho - A(1));
+  pi[0] = s[0] - rho * u[0] * u[0] - iso;
+  pi[1] = s[1] - rho * u[0] * u[1];
+  pi[2] = s[2] - rho * u[0] * u[2];
+  pi[3] = s[3] - rho * u[1] * u[1] - iso;
+  pi[4] = s[4] - rho * u[1] * u[2];
+  pi[5] = s[5] - rho * u[2] * u[2] - iso;
+}
+
+// InnerEdgeDensity3D / InnerCornerDensity3D (momenta/elements.h:432-482):
+// the mean of velocityBMRho over the non-zero axes of the outward normal; an
+// edge's axes are plane+1 and plane+2 with plane its zero axis
+// (boundary/setBoundary3D.h:156-197)
+template <typename A>
+__device__ __forceinline__ A inner_density(const A (&f)[Q], const int (&N)[3], const A (&u)[3])
+{
+  if (N[0] != 0 && N[1] != 0 && N[2] != 0) {
+    return (velocity_bm_rho(f, 0, N[0], u[0]) + velocity_bm_rho(f, 1, N[1], u[1]) +
+            velocity_bm_rho(f, 2, N[2], u[2])) / A(3);
+  }
+  const int plane = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+  const int a1 = (plane + 1) % 3, a2 = (plane + 2) % 3;
+  return (velocity_bm_rho(f, a1, N[a1], u[a1]) + velocity_bm_rho(f, a2, N[a2], u[a2])) / A(2);
+}
+
+// CombinedRLBdynamics<BGK, InnerEdge / InnerCornerVelocityTuple3D>::collide:
+// the stress of InnerEdgeStress3D replaces the population entering from both
+// walls by bounce-back of its non-equilibrium part (elements.h:1871-1892);
+// an inner corner's entering population -N does not exist in D3Q19
+// (elements.h:1843-1869), so its stress is the bulk one
+template <typename A>
+__device__ __forceinline__ void inner_velocity(A (&f)[Q], const int (&N)[3], const A (&u)[3], A om, A& rho_out)
+{
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  A rho = inner_density(f, N, u);
+  A g[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) g[i] = f[i];
+  if (N[0] == 0 || N[1] == 0 || N[2] == 0) {
+    const int plane = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+    const int a1 = (plane + 1) % 3, a2 = (plane + 2) % 3;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      if (CA(i, a1) == -N[a1] && CA(i, a2) == -N[a2]) {
+        const int o = i <= 9 ? i + 9 : i - 9;
+        g[i] = f[o] - feq_i(o, rho, u, usqr) + feq_i(i, rho, u, usqr);
+      }
+    }
+  }
+  A pi[6];
+  bulk_stress(g, rho, u, pi);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi);
+  }
+  rho = inner_density(f, N, u);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = f[i] + om * (feq_i(i, rho, u, usqr) - f[i]);
+  }
+  rho_out = rho;
+}
+
+// FixedPressureMomentum::computeU (momenta/elements.h:668-691): the wall's
+// tangential velocity, the normal component from the known populations
+template <typename A>
+__device__ __forceinline__ void pressure_u(const A (&f)[Q], int dir, int orient, A rho, const A (&uw)[3],
+                                           A (&u)[3])
+{
+  A on_wall = A(0), normal = A(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int c = CA(i, dir);
+    if (c == 0) {
+      on_wall += f[i];
+    } else if (c == orient) {
+      normal += f[i];
+    }
+  }
+  u[0] = uw[0];
+  u[1] = uw[1];
+  u[2] = uw[2];
+  u[dir] = A(orient) * ((A(2) * normal + on_wall + A(1)) / rho - A(1));
+}
+
+// CombinedRLBdynamics<BGK, RegularizedPressureBoundaryTuple<dir, orient>>::collide
+template <typename A>
+__device__ __forceinline__ void local_pressure(A (&f)[Q], int dir, int orient, A rho, const A (&uw)[3], A om,
+                                               A (&u)[3])
+{
+  pressure_u(f, dir, orient, rho, uw, u);
+  A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  A pi[6] = {A(0), A(0), A(0), A(0), A(0), A(0)};
+#pragma unroll
+  for (int i = 1; i < Q; ++i) {
+    const int c = CA(i, dir);
+    if (c == 0 || c == orient) {
+      const A w = (c == 0 ? A(1) : A(2)) * (f[i] - feq_i(i, rho, u, usqr));
+      const A cx = A(CX(i)), cy = A(CY(i)), cz = A(CZ(i));
+      pi[0] += cx * cx * w;
+      pi[1] += cx * cy * w;
+      pi[2] += cx * cz * w;
+      pi[3] += cy * cy * w;
+      pi[4] += cy * cz * w;
+      pi[5] += cz * cz * w;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi);
+  }
+  pressure_u(f, dir, orient, rho, uw, u);
+  usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = f[i] + om * (feq_i(i, rho, u, usqr) - f[i]);
+  }
+}
+
+// the flat normal of a cell: direction and orientation
+__device__ __forceinline__ void flat_normal(uint32_t code, int& dir, int& orient)
+{
+  dir = 0;
+  orient = normal_component(code, 0);
+  if (normal_component(code, 1) != 0) {
+    dir = 1;
+    orient = normal_component(code, 1);
+  } else if (normal_component(code, 2) != 0) {
+    dir = 2;
+    orient = normal_component(code, 2);
+  }
+}
+
+// the boundary kinds that read the discrete normal (the masked kernel with
+// boundary kinds); returns true: all yield a statistic
 template <typename A>
 __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
                                                         const StepArgs<A>& a, A& st_rho, A& st_usqr)
@@ -616,17 +767,22 @@ __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntr
   const A u[3] = {wall_component(a, e, n, 0), wall_component(a, e, n, 1), wall_component(a, e, n, 2)};
   A rho;
   if (e.kind == OLBGFX_LOCAL_VELOCITY) {
-    const uint32_t code = a.normal[n];
-    int dir = 0;
-    int orient = normal_component(code, 0);
-    if (normal_component(code, 1) != 0) {
-      dir = 1;
-      orient = normal_component(code, 1);
-    } else if (normal_component(code, 2) != 0) {
-      dir = 2;
-      orient = normal_component(code, 2);
-    }
+    int dir, orient;
+    flat_normal(a.normal[n], dir, orient);
     local_velocity(f, dir, orient, u, e.omega, rho);
+  } else if (e.kind == OLBGFX_LOCAL_PRESSURE) {
+    int dir, orient;
+    flat_normal(a.normal[n], dir, orient);
+    rho = a.cell_rho ? a.cell_rho[n] : e.wall_rho;
+    A up[3];
+    local_pressure(f, dir, orient, rho, u, e.omega, up);
+    st_rho = rho;
+    st_usqr = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+    return true;
+  } else if (e.kind == OLBGFX_INNER_EDGE_VELOCITY || e.kind == OLBGFX_INNER_CORNER_VELOCITY) {
+    const uint32_t code = a.normal[n];
+    const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
+    inner_velocity(f, N, u, e.omega, rho);
   } else {
     // BulkDensity + FixedVelocityMomentumGeneric, BGK (collision.h:57-80)
     A s = A(0);
@@ -804,7 +960,14 @@ __device__ typename Storage<S>::A momenta_rho(const StepArgs<typename Storage<S>
   A f[Q];
   load_cell<S>(a, m, f);
   const DynEntry<A>& e = a.table[a.mask[m]];
-  if (e.kind == OLBGFX_LOCAL_VELOCITY) {
+  switch (e.kind) {
+  case OLBGFX_NO_DYNAMICS:
+    return A(1);  // OneDensity
+  case OLBGFX_BOUNCE_BACK:
+  case OLBGFX_BOUNCE_BACK_VELOCITY:
+  case OLBGFX_LOCAL_PRESSURE:
+    return a.cell_rho ? a.cell_rho[m] : e.wall_rho;  // FixedDensity
+  case OLBGFX_LOCAL_VELOCITY: {
     const uint32_t code = a.normal[m];
     for (int ax = 0; ax < 3; ++ax) {
       const int o = normal_component(code, ax);
@@ -812,6 +975,17 @@ __device__ typename Storage<S>::A momenta_rho(const StepArgs<typename Storage<S>
         return velocity_bm_rho(f, ax, o, wall_component(a, e, m, ax));
       }
     }
+    break;
+  }
+  case OLBGFX_INNER_EDGE_VELOCITY:
+  case OLBGFX_INNER_CORNER_VELOCITY: {
+    const uint32_t code = a.normal[m];
+    const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
+    const A u[3] = {wall_component(a, e, m, 0), wall_component(a, e, m, 1), wall_component(a, e, m, 2)};
+    return inner_density(f, N, u);
+  }
+  default:
+    break;
   }
   A s = A(0);
 #pragma unroll
@@ -825,9 +999,29 @@ __device__ void momenta_u(const StepArgs<typename Storage<S>::A>& a, uint32_t m,
 {
   using A = typename Storage<S>::A;
   const DynEntry<A>& e = a.table[a.mask[m]];
-  if (e.kind == OLBGFX_LOCAL_VELOCITY || e.kind == OLBGFX_FIXED_VELOCITY) {
+  switch (e.kind) {
+  case OLBGFX_NO_DYNAMICS:
+  case OLBGFX_BOUNCE_BACK:
+    u[0] = u[1] = u[2] = A(0);  // ZeroMomentum
+    return;
+  case OLBGFX_LOCAL_VELOCITY:
+  case OLBGFX_FIXED_VELOCITY:
+  case OLBGFX_INNER_EDGE_VELOCITY:
+  case OLBGFX_INNER_CORNER_VELOCITY:
+  case OLBGFX_BOUNCE_BACK_VELOCITY:
     for (int ax = 0; ax < 3; ++ax) u[ax] = wall_component(a, e, m, ax);
     return;
+  case OLBGFX_LOCAL_PRESSURE: {
+    A f[Q];
+    load_cell<S>(a, m, f);
+    int dir, orient;
+    flat_normal(a.normal[m], dir, orient);
+    const A uw[3] = {wall_component(a, e, m, 0), wall_component(a, e, m, 1), wall_component(a, e, m, 2)};
+    pressure_u(f, dir, orient, a.cell_rho ? a.cell_rho[m] : e.wall_rho, uw, u);
+    return;
+  }
+  default:
+    break;
   }
   A f[Q];
   load_cell<S>(a, m, f);
@@ -918,12 +1112,13 @@ __global__ void __launch_bounds__(256) k_velocity_post(const StepArgs<typename S
 
 // ---------------------------------------------------------------------------
 // Cell::computeRhoU of every padded cell through the momenta of its dynamics
-// (dynamics/momenta/elements.h): NoDynamics OneDensity and ZeroMomentum
-// (dynamics.h:52-62); BounceBack FixedDensity (the cell's RHO, 1 unless
-// defined) and ZeroMomentum (dynamics.h:557-568); BounceBackVelocity
-// FixedDensity and FixedVelocityMomentum; LocalVelocity faces the
-// velocity-BM density and the wall velocity, edges and corners the bulk
-// density and the wall velocity; forced dynamics add F/2; the rest are bulk.
+// (dynamics/momenta/elements.h, through momenta_rho and momenta_u):
+// NoDynamics OneDensity and ZeroMomentum (dynamics.h:52-62); BounceBack
+// FixedDensity (the cell's RHO, 1 unless defined) and ZeroMomentum
+// (dynamics.h:557-568); BounceBackVelocity FixedDensity and
+// FixedVelocityMomentum; the LocalVelocity kinds their boundary densities and
+// the wall velocity; LocalPressure its fixed density and pressure velocity;
+// forced dynamics add F/2; the rest are bulk.
 template <typename S, typename H>
 __global__ void __launch_bounds__(256) k_cell_rho_u(const StepArgs<typename Storage<S>::A> a, H* rho_out,
                                                     H* u_out)
@@ -933,22 +1128,33 @@ __global__ void __launch_bounds__(256) k_cell_rho_u(const StepArgs<typename Stor
   if (n >= a.count) {
     return;
   }
-  const DynEntry<A>& e = a.table[a.mask[n]];
   A rho, u[3];
-  if (e.kind == OLBGFX_NO_DYNAMICS) {
-    rho = A(1);
-    u[0] = u[1] = u[2] = A(0);
-  } else if (e.kind == OLBGFX_BOUNCE_BACK || e.kind == OLBGFX_BOUNCE_BACK_VELOCITY) {
-    rho = a.cell_rho ? a.cell_rho[n] : e.wall_rho;
-    for (int ax = 0; ax < 3; ++ax) {
-      u[ax] = e.kind == OLBGFX_BOUNCE_BACK ? A(0) : wall_component(a, e, n, ax);
-    }
-  } else {
-    rho = momenta_rho<S>(a, n);
-    momenta_u<S>(a, n, u);
-  }
+  rho = momenta_rho<S>(a, n);
+  momenta_u<S>(a, n, u);
   rho_out[n] = (H)rho;
   for (int ax = 0; ax < 3; ++ax) u_out[3 * (size_t)n + ax] = (H)u[ax];
+}
+
+// ---------------------------------------------------------------------------
+// PostStream overlap exchange for the post-processors (boundary/setBoundary.h:
+// 110-118, addPoints2CommBC): before they run, the periodic padding cells
+// they read receive their core image's streamed populations.  pairs holds
+// (padding cell, image) ids; padding cells are never images, so the copies
+// do not race.
+template <typename S>
+__global__ void __launch_bounds__(256) k_refresh(const StepArgs<typename Storage<S>::A> a, const uint32_t* pairs,
+                                                 uint32_t npairs)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs) {
+    return;
+  }
+  const uint32_t g = pairs[2 * t], m = pairs[2 * t + 1];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    *ring_at<S>(a.pop, a.stride, i, ring_slot(g, a.shift[i], a.count)) =
+        *ring_at<S>(a.pop, a.stride, i, ring_slot(m, a.shift[i], a.count));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,6 +1549,8 @@ struct olbgfx_lattice {
   uint8_t* normal_dev = nullptr;     // code 9(nx+1)+3(ny+1)+(nz+1) per cell
   uint32_t* post_dev = nullptr;
   uint32_t post_edges = 0, post_corners = 0;
+  uint32_t* refresh_dev = nullptr;   // (periodic padding cell, image) pairs the edge/corner processors read
+  uint32_t refresh_n = 0;
   bool boundary_kinds = false;       // some core cell has a LocalVelocity kind
   // Bouzidi boundaries: the cells that run the post-processor and their
   // distances, q[(i - 1) * bz_n + k] for direction i of cell k
@@ -1456,6 +1664,36 @@ void launch_mode(int mode, dim3 grid, dim3 block, size_t lds, hipStream_t stream
 
 olbgfx_status prepare_boundary(olbgfx_lattice* L);
 
+// PostStream processors on the populations at the lattice's current offsets:
+// the overlap exchange (exchange = true, as SuperLattice::collideAndStream
+// does before them), Bouzidi (priority -1), LocalVelocity edges (priority 0),
+// then corners (priority 1)
+template <typename S>
+olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a, hipStream_t stream,
+                          bool exchange)
+{
+  using A = typename Storage<S>::A;
+  for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
+  if (exchange && L->refresh_n > 0) {
+    k_refresh<S><<<(L->refresh_n + 255) / 256, 256, 0, stream>>>(a, L->refresh_dev, L->refresh_n);
+    HIP_TRY(hipGetLastError());
+  }
+  if (L->bz_n > 0) {
+    k_bouzidi<S><<<(L->bz_n + 255) / 256, 256, 0, stream>>>(a, L->bz_cells, (const A*)L->bz_q, L->bz_n);
+    HIP_TRY(hipGetLastError());
+  }
+  if (L->post_edges) {
+    k_velocity_post<S><<<(L->post_edges + 255) / 256, 256, 0, stream>>>(a, L->post_dev, L->post_edges);
+    HIP_TRY(hipGetLastError());
+  }
+  if (L->post_corners) {
+    k_velocity_post<S><<<(L->post_corners + 255) / 256, 256, 0, stream>>>(a, L->post_dev + L->post_edges,
+                                                                         L->post_corners);
+    HIP_TRY(hipGetLastError());
+  }
+  return OLBGFX_OK;
+}
+
 template <typename S>
 olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream)
 {
@@ -1497,23 +1735,8 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
       const uint64_t s = (uint64_t)L->shift[i] + (uint64_t)L->dmod[i];
       L->shift[i] = (uint32_t)(s >= L->count ? s - L->count : s);
     }
-    // PostStream: Bouzidi (priority -1), LocalVelocity edges (priority 0),
-    // then corners (priority 1)
-    if (L->bz_n > 0) {
-      for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
-      k_bouzidi<S><<<(L->bz_n + 255) / 256, 256, 0, stream>>>(a, L->bz_cells, (const A*)L->bz_q, L->bz_n);
-      HIP_TRY(hipGetLastError());
-    }
-    if (L->post_edges + L->post_corners > 0) {
-      for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
-      if (L->post_edges) {
-        k_velocity_post<S><<<(L->post_edges + 255) / 256, 256, 0, stream>>>(a, L->post_dev, L->post_edges);
-      }
-      if (L->post_corners) {
-        k_velocity_post<S><<<(L->post_corners + 255) / 256, 256, 0, stream>>>(a, L->post_dev + L->post_edges,
-                                                                             L->post_corners);
-      }
-      HIP_TRY(hipGetLastError());
+    if (launch_post<S>(L, a, stream, true) != OLBGFX_OK) {
+      return OLBGFX_ERR_HIP;
     }
   }
   if (steps > 0) {
@@ -1681,8 +1904,14 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
             nz += N[a] != 0;
           }
         }
-        if (kind == OLBGFX_LOCAL_VELOCITY && nz != 1) {
-          return fail(OLBGFX_ERR_INVALID, "a LOCAL_VELOCITY cell needs a flat discrete normal");
+        if ((kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE) && nz != 1) {
+          return fail(OLBGFX_ERR_INVALID, "a LOCAL_VELOCITY or LOCAL_PRESSURE cell needs a flat discrete normal");
+        }
+        if (kind == OLBGFX_INNER_EDGE_VELOCITY && nz != 2) {
+          return fail(OLBGFX_ERR_INVALID, "an INNER_EDGE_VELOCITY cell needs a normal with two non-zero entries");
+        }
+        if (kind == OLBGFX_INNER_CORNER_VELOCITY && nz != 3) {
+          return fail(OLBGFX_ERR_INVALID, "an INNER_CORNER_VELOCITY cell needs a normal with three non-zero entries");
         }
         if (kind == OLBGFX_FIXED_VELOCITY && nz >= 2) {
           for (int a = 0; a < 3; ++a) {
@@ -1695,6 +1924,80 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
         }
       }
     }
+  }
+  // The reference runs the processors of one priority one after another in
+  // ascending cell id; here they run concurrently.  They agree unless one
+  // processor reads the density of another cell of the same list (an edge
+  // two cells inward of an edge, a corner of a corner), which the staircase
+  // walls seen so far never produce: such a configuration is refused.
+  {
+    std::vector<uint8_t> in(L->count, 0);
+    for (uint32_t n : edges) in[n] = 1;
+    for (uint32_t n : corners) in[n] = 2;
+    const int64_t step[3] = {(int64_t)L->py * L->pz, (int64_t)L->pz, 1};
+    for (int list = 1; list <= 2; ++list) {
+      for (uint32_t n : (list == 1 ? edges : corners)) {
+        int N[3];
+        for (int ax = 0; ax < 3; ++ax) N[ax] = L->host_normals[3 * (size_t)n + ax];
+        for (int ax = 0; ax < 3; ++ax) {
+          if (N[ax] == 0) continue;
+          for (int k = 1; k <= 2; ++k) {
+            const int64_t m = (int64_t)n - (int64_t)k * N[ax] * step[ax];
+            if (in[(size_t)m] == list) {
+              return fail(OLBGFX_ERR_UNSUPPORTED,
+                          "a LocalVelocity edge or corner processor reads the density of another one of its kind");
+            }
+          }
+        }
+      }
+    }
+  }
+  // periodic padding cells within reach (2) of an edge or corner processor
+  std::vector<uint32_t> pairs;
+  if (D.periodic[0] || D.periodic[1] || D.periodic[2]) {
+    std::vector<uint8_t> seen(L->count, 0);
+    const int ext[3] = {D.nx, D.ny, D.nz};
+    auto visit = [&](uint32_t n) {
+      const int c[3] = {(int)(n / (L->py * L->pz)) - D.padding, (int)(n / L->pz % L->py) - D.padding,
+                        (int)(n % L->pz) - D.padding};
+      for (int dx = -2; dx <= 2; ++dx)
+        for (int dy = -2; dy <= 2; ++dy)
+          for (int dz = -2; dz <= 2; ++dz) {
+            int g[3] = {c[0] + dx, c[1] + dy, c[2] + dz}, m[3];
+            bool pad = false, ok = true;
+            for (int ax = 0; ax < 3; ++ax) {
+              m[ax] = g[ax];
+              if (g[ax] < 0 || g[ax] >= ext[ax]) {
+                pad = true;
+                if (!D.periodic[ax] || g[ax] < -D.padding || g[ax] >= ext[ax] + D.padding) {
+                  ok = false;
+                } else {
+                  m[ax] = g[ax] < 0 ? g[ax] + ext[ax] : g[ax] - ext[ax];
+                }
+              }
+            }
+            if (!pad || !ok) continue;
+            const uint32_t gn = ((uint32_t)(g[0] + D.padding) * L->py + (uint32_t)(g[1] + D.padding)) * L->pz +
+                                (uint32_t)(g[2] + D.padding);
+            if (seen[gn]) continue;
+            seen[gn] = 1;
+            const uint32_t mn = ((uint32_t)(m[0] + D.padding) * L->py + (uint32_t)(m[1] + D.padding)) * L->pz +
+                                (uint32_t)(m[2] + D.padding);
+            pairs.push_back(gn);
+            pairs.push_back(mn);
+          }
+    };
+    for (uint32_t n : edges) visit(n);
+    for (uint32_t n : corners) visit(n);
+  }
+  if (L->refresh_dev) {
+    HIP_TRY(hipFree(L->refresh_dev));
+    L->refresh_dev = nullptr;
+  }
+  L->refresh_n = (uint32_t)(pairs.size() / 2);
+  if (!pairs.empty()) {
+    HIP_TRY(hipMalloc(&L->refresh_dev, sizeof(uint32_t) * pairs.size()));
+    HIP_TRY(hipMemcpy(L->refresh_dev, pairs.data(), sizeof(uint32_t) * pairs.size(), hipMemcpyHostToDevice));
   }
   if (L->post_dev) {
     HIP_TRY(hipFree(L->post_dev));
@@ -1815,6 +2118,7 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->cell_force);
   hipFree(L->normal_dev);
   hipFree(L->post_dev);
+  hipFree(L->refresh_dev);
   hipFree(L->bz_cells);
   hipFree(L->bz_q);
   hipFree(L->stats_partial);
@@ -1862,7 +2166,7 @@ olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_d
   if (id >= OLBGFX_MAX_DYNAMICS) {
     return fail(OLBGFX_ERR_INVALID, "dynamics id out of range");
   }
-  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_SMAGORINSKY_BGK) {
+  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_LOCAL_PRESSURE) {
     return fail(OLBGFX_ERR_UNSUPPORTED, "dynamics kind not provided by the gfx950 platform");
   }
   L->table[id] = *dyn;
@@ -2136,6 +2440,24 @@ olbgfx_status olbgfx_collide_and_stream(olbgfx_lattice* L, uint32_t steps, void*
   return dispatch_storage(L, [&](auto* tag) {
     using S = std::remove_pointer_t<decltype(tag)>;
     return launch_steps<S>(L, steps, s);
+  });
+}
+
+olbgfx_status olbgfx_post_process(olbgfx_lattice* L, void* stream)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (prepare_boundary(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    StepArgs<typename Storage<S>::A> a;
+    fill_args(L, a, false);
+    // BlockLattice::postProcess runs without the super-level exchange
+    return launch_post<S>(L, a, s, false);
   });
 }
 

@@ -474,6 +474,41 @@ public:
     _dirty = true;
   }
 
+  /// LocalPressure on the flat faces of `material` (boundary::set<boundary::
+  /// LocalPressure>, boundary/localPressure3D.h:47-58): regularised faces
+  /// with the density the RHO field (defineRho) or 1 gives and the velocity's
+  /// normal component from the known populations; any other boundary shape
+  /// throws std::runtime_error, as the reference does
+  void setLocalPressure(SuperGeometry<T>& geometry, int material)
+  {
+    const int face = slotFor(OLBGFX_LOCAL_PRESSURE);
+    if (_normals.empty()) {
+      _normals.assign(3 * _ncells, 0);
+    }
+    geometry.forCore([&](int x, int y, int z) {
+      if (geometry.get(x, y, z) != material) return;
+      const auto [type, n] = discreteNormal(geometry, x, y, z);
+      if (type != NormalType::Flat) {
+        throw std::runtime_error("No valid discrete normal found. This BC is not suited for curved walls.");
+      }
+      const CellID c = geometry.getCellId(x, y, z);
+      _slotOfCell[c] = (std::uint8_t)face;
+      for (int a = 0; a < 3; ++a) _normals[3 * (std::size_t)c + a] = (std::int8_t)n[a];
+    });
+    _dirty = true;
+  }
+
+  /// the RHO field (FixedDensity momenta) on the cells of `material`
+  void defineRho(SuperGeometry<T>& geometry, int material, T rho)
+  {
+    auto& field = fieldOf(OLBGFX_FIELD_RHO);
+    forMaterial(geometry, material, [&](CellID c) {
+      field.values[c] = rho;
+      field.set[c] = 1;
+    });
+    _dirty = true;
+  }
+
   /// set a dynamics parameter on every dynamics that has it
   template <typename PARAMETER>
   void setParameter(T value)
@@ -558,6 +593,11 @@ public:
                                   kv.second.uniform ? nullptr : kv.second.values.data()));
     }
     check(olbgfx_set_statistics(handle(), _statistics ? 1 : 0));
+    if (!_initialized) {
+      // BlockLattice::initialize runs the PostStream processors once
+      // (core/blockLattice.hh:56-67)
+      check(olbgfx_post_process(handle(), nullptr));
+    }
     _dirty = false;
     _initialized = true;
   }
@@ -761,11 +801,14 @@ private:
     return (int)_slots.size() - 1;
   }
 
+  // RHO holds one value per cell (1 where undefined, FixedDensity's
+  // initial value), the vector fields three
   Field& fieldOf(int32_t id)
   {
     Field& f = _fields[id];
     if (f.values.empty()) {
-      f.values.assign(3 * _ncells, T(0));
+      const bool scalar = id == OLBGFX_FIELD_RHO;
+      f.values.assign((scalar ? 1 : 3) * _ncells, T(scalar ? 1 : 0));
       f.set.assign(_ncells, 0);
     }
     return f;
@@ -867,11 +910,16 @@ struct BounceBack {
 /// boundary::LocalVelocity<T,D,BGKdynamics> (boundary/localVelocity3D.h:36-120)
 struct LocalVelocity {
 };
+/// boundary::LocalPressure<T,D,BGKdynamics> (boundary/localPressure3D.h:36-62)
+struct LocalPressure {
+};
 template <typename BC, typename T, typename DESCRIPTOR>
 void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material)
 {
   if constexpr (std::is_same_v<BC, LocalVelocity>) {
     lattice.setLocalVelocity(geometry, material);
+  } else if constexpr (std::is_same_v<BC, LocalPressure>) {
+    lattice.setLocalPressure(geometry, material);
   } else {
     lattice.template defineDynamics<BC::template dynamics>(geometry, material);
   }

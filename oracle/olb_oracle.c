@@ -92,6 +92,9 @@ enum {
   KIND_LOCAL_VELOCITY = 7,   /* flat LocalVelocity face: CombinedRLB<BGK, RegularizedVelocityBoundaryTuple> */
   KIND_FIXED_VELOCITY = 8,   /* LocalVelocity edge/corner: BGK<FixedVelocityBoundaryTuple> + post-processor */
   KIND_SMAGORINSKY_BGK = 9,  /* SmagorinskyBGKdynamics */
+  KIND_INNER_EDGE = 10,      /* LocalVelocity inner edge: CombinedRLB<BGK, InnerEdgeVelocityTuple3D> */
+  KIND_INNER_CORNER = 11,    /* LocalVelocity inner corner: CombinedRLB<BGK, InnerCornerVelocityTuple3D> */
+  KIND_LOCAL_PRESSURE = 12,  /* LocalPressure face: CombinedRLB<BGK, RegularizedPressureBoundaryTuple> */
 };
 
 enum { STORE_NATIVE = 0, STORE_F16 = 1, STORE_BF16 = 2 };
@@ -557,6 +560,86 @@ static real pi_to_fneq(int i, const real pi[6])
 }
 
 /* one cell; returns 1 if the cell contributes to the statistics */
+/* lbm<D>::computeStress (dynamics/lbm.h:334-352): sum c_a c_b f - rho u_a u_b
+ * - (rho - 1)/3 delta_ab, on shifted populations */
+static void bulk_stress(const real f[Q], real rho, const real u[3], real pi[6])
+{
+  int k = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int b = a; b < 3; ++b) {
+      real v = 0;
+      for (int i = 0; i < Q; ++i) v += (real)(C[i][a] * C[i][b]) * f[i];
+      v -= rho * u[a] * u[b];
+      if (a == b) v -= (real)1 / (real)3 * (rho - (real)1);
+      pi[k++] = v;
+    }
+  }
+}
+
+/* InnerEdgeDensity3D / InnerCornerDensity3D (momenta/elements.h:432-482):
+ * the mean of velocityBMRho over the non-zero axes of the outward normal */
+static real inner_density(const real f[Q], const int N[3], const real u[3])
+{
+  real s = 0;
+  int k = 0;
+  if (N[0] != 0 && N[1] != 0 && N[2] != 0) {
+    const real rx = velocity_bm_rho(f, 0, N[0], u);
+    const real ry = velocity_bm_rho(f, 1, N[1], u);
+    const real rz = velocity_bm_rho(f, 2, N[2], u);
+    return (rx + ry + rz) / (real)3;
+  }
+  /* edge: plane = the zero axis, normal1 = N[plane+1], normal2 = N[plane+2]
+   * (setBoundary3D.h:156-197) */
+  int plane = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+  const int a1 = (plane + 1) % 3, a2 = (plane + 2) % 3;
+  s = velocity_bm_rho(f, a1, N[a1], u) + velocity_bm_rho(f, a2, N[a2], u);
+  (void)k;
+  return s / (real)2;
+}
+
+/* InnerEdgeStress3D / InnerCornerStress3D (momenta/elements.h:1844-1898):
+ * the populations entering from both walls are replaced by bounce-back of
+ * their non-equilibrium part, then the bulk stress */
+static void inner_stress(const real f[Q], const int N[3], real rho, const real u[3], real pi[6])
+{
+  real g[Q];
+  for (int i = 0; i < Q; ++i) g[i] = f[i];
+  const real usqr = norm_sqr(u);
+  if (!(N[0] != 0 && N[1] != 0 && N[2] != 0)) {
+    const int plane = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+    const int a1 = (plane + 1) % 3, a2 = (plane + 2) % 3;
+    for (int i = 0; i < Q; ++i) {
+      if (C[i][a1] == -N[a1] && C[i][a2] == -N[a2]) {
+        const int o = opp(i);
+        g[i] = g[o] - eq_second_order(o, rho, u, usqr) + eq_second_order(i, rho, u, usqr);
+      }
+    }
+  }
+  /* an inner corner's entering population c = -N does not exist in D3Q19 */
+  bulk_stress(g, rho, u, pi);
+}
+
+/* FixedPressureMomentum::computeU (momenta/elements.h:653-691): the wall's
+ * tangential velocity, the normal component from the known populations */
+static void pressure_u(const real f[Q], int dir, int orient, real rho, const real uw[3], real u[3])
+{
+  real on_wall = 0, normal = 0;
+  for (int i = 0; i < Q; ++i) if (C[i][dir] == 0) on_wall += f[i];
+  for (int i = 0; i < Q; ++i) if (C[i][dir] == orient) normal += f[i];
+  for (int a = 0; a < 3; ++a) u[a] = uw[a];
+  u[dir] = (real)orient * (((real)2 * normal + on_wall + (real)1) / rho - (real)1);
+}
+
+static real wall_density(const lattice_t* L, const dyn_t* D, uint64_t n)
+{
+  return L->cell_rho ? L->cell_rho[n] : D->wall_rho;
+}
+
+static void cell_normal(const lattice_t* L, uint64_t n, int N[3])
+{
+  for (int a = 0; a < 3; ++a) N[a] = L->normals ? L->normals[3 * n + a] : 0;
+}
+
 static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho, real* st_usqr)
 {
   const dyn_t* D = &L->table[L->mask[n]];
@@ -606,16 +689,7 @@ static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho,
      * of lbm.h:334-352 and the norm of lbm.h:555-564 */
     rho_u(f, &rho, u);
     real pi[6];
-    int k = 0;
-    for (int a = 0; a < 3; ++a) {
-      for (int b = a; b < 3; ++b) {
-        real v = 0;
-        for (int i = 0; i < Q; ++i) v += (real)(C[i][a] * C[i][b]) * f[i];
-        v -= rho * u[a] * u[b];
-        if (a == b) v -= (real)1 / (real)3 * (rho - (real)1);
-        pi[k++] = v;
-      }
-    }
+    bulk_stress(f, rho, u, pi);
     real norm_sqr_pi = pi[0] * pi[0] + (real)(2. * pi[1] * pi[1]) + pi[2] * pi[2];
     norm_sqr_pi += pi[2] * pi[2] + pi[3] * pi[3] + (real)(2. * pi[4] * pi[4]) + pi[5] * pi[5];
     const real c = D->smagorinsky;
@@ -642,6 +716,41 @@ static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho,
     rho = velocity_bm_rho(f, dir, orient, uu);
     bgk(f, rho, uu, D->omega);
     *st_rho = rho; *st_usqr = usqr;
+    return 1;
+  }
+  case KIND_INNER_EDGE:
+  case KIND_INNER_CORNER: {
+    /* CombinedRLBdynamics<BGK, InnerEdge/InnerCornerVelocityTuple3D>
+     * (boundary/localVelocity3D.h:86-99) */
+    int N[3];
+    cell_normal(L, n, N);
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    real pi[6];
+    rho = inner_density(f, N, uu);
+    inner_stress(f, N, rho, uu, pi);
+    const real usqr = norm_sqr(uu);
+    for (int i = 0; i < Q; ++i) f[i] = eq_second_order(i, rho, uu, usqr) + pi_to_fneq(i, pi);
+    rho = inner_density(f, N, uu);
+    bgk(f, rho, uu, D->omega);
+    *st_rho = rho; *st_usqr = usqr;
+    return 1;
+  }
+  case KIND_LOCAL_PRESSURE: {
+    /* CombinedRLBdynamics<BGK, RegularizedPressureBoundaryTuple<dir,orient>>
+     * (boundary/localPressure3D.h:47-58, momenta/aliases.h:152-159) */
+    int dir = 0, orient = 0;
+    if (!flat_normal(L, n, &dir, &orient)) return 0;
+    const real* uw = wall_velocity(L, D, n);
+    real pi[6], uu[3];
+    rho = wall_density(L, D, n);
+    pressure_u(f, dir, orient, rho, uw, uu);
+    boundary_stress(f, dir, orient, rho, uu, pi);
+    real usqr = norm_sqr(uu);
+    for (int i = 0; i < Q; ++i) f[i] = eq_second_order(i, rho, uu, usqr) + pi_to_fneq(i, pi);
+    pressure_u(f, dir, orient, rho, uw, uu);
+    bgk(f, rho, uu, D->omega);
+    *st_rho = rho; *st_usqr = norm_sqr(uu);
     return 1;
   }
   case KIND_FIXED_VELOCITY: {
@@ -722,6 +831,59 @@ static void periodic_copy(lattice_t* L)
   }
 }
 
+/* PostStream overlap exchange of the post-processor neighbourhood
+ * (boundary/setBoundary3D.h:57-110 addPoints2CommBC requests the populations
+ * of the padding cells near the boundary): the padding cells within reach
+ * (2) of an edge or corner processor -- the ones those processors read --
+ * whose periodic image lies in the core get the image's streamed
+ * populations, before any post-processor runs. */
+static int near_processor(const lattice_t* L, const int g[3])
+{
+  if (!L->normals) return 0;
+  for (int dx = -2; dx <= 2; ++dx) {
+    for (int dy = -2; dy <= 2; ++dy) {
+      for (int dz = -2; dz <= 2; ++dz) {
+        const int c[3] = {g[0] + dx, g[1] + dy, g[2] + dz};
+        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= L->nx || c[1] >= L->ny || c[2] >= L->nz) continue;
+        const uint64_t n = (uint64_t)((c[0] + L->pad) * L->proj[0] + (c[1] + L->pad) * L->proj[1] + (c[2] + L->pad));
+        if (L->table[L->mask[n]].kind != KIND_FIXED_VELOCITY) continue;
+        const int nz = (L->normals[3 * n] != 0) + (L->normals[3 * n + 1] != 0) + (L->normals[3 * n + 2] != 0);
+        if (nz >= 2) return 1;
+      }
+    }
+  }
+  return 0;
+}
+
+static void periodic_refresh(lattice_t* L)
+{
+  const int n3[3] = {L->nx, L->ny, L->nz};
+  const int p = L->pad;
+  for (int x = -p; x < L->nx + p; ++x) {
+    for (int y = -p; y < L->ny + p; ++y) {
+      for (int z = -p; z < L->nz + p; ++z) {
+        int g[3] = {x, y, z};
+        int img[3];
+        int outside = 0, ok = 1;
+        for (int a = 0; a < 3; ++a) {
+          img[a] = g[a];
+          if (g[a] < 0 || g[a] >= n3[a]) {
+            outside = 1;
+            if (!L->periodic[a]) { ok = 0; break; }
+            img[a] = g[a] < 0 ? g[a] + n3[a] : g[a] - n3[a];
+          }
+        }
+        if (!outside || !ok || !near_processor(L, g)) continue;
+        uint64_t gn = (uint64_t)((g[0] + p) * L->proj[0] + (g[1] + p) * L->proj[1] + (g[2] + p));
+        uint64_t in = (uint64_t)((img[0] + p) * L->proj[0] + (img[1] + p) * L->proj[1] + (img[2] + p));
+        for (int i = 0; i < Q; ++i) {
+          L->ring[i][slot(L, i, gn)] = L->ring[i][slot(L, i, in)];
+        }
+      }
+    }
+  }
+}
+
 /* cpu/sisd/column.h:193-204 rotate(d_i): shift -= d_i (mod count) */
 static void stream(lattice_t* L)
 {
@@ -746,12 +908,30 @@ static real momenta_rho(const lattice_t* L, uint64_t n)
   const dyn_t* D = &L->table[L->mask[n]];
   real f[Q], rho, u[3];
   load_cell(L, n, f);
-  if (D->kind == KIND_LOCAL_VELOCITY) {
+  switch (D->kind) {
+  case KIND_NONE:
+    return (real)1;                      /* OneDensity */
+  case KIND_BOUNCE_BACK:
+  case KIND_BOUNCE_BACK_VELOCITY:
+  case KIND_LOCAL_PRESSURE:
+    return wall_density(L, D, n);        /* FixedDensity */
+  case KIND_LOCAL_VELOCITY: {
     int dir = 0, orient = 0;
     flat_normal(L, n, &dir, &orient);
     const real* uw = wall_velocity(L, D, n);
     real uu[3] = {uw[0], uw[1], uw[2]};
     return velocity_bm_rho(f, dir, orient, uu);
+  }
+  case KIND_INNER_EDGE:
+  case KIND_INNER_CORNER: {
+    int N[3];
+    cell_normal(L, n, N);
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    return inner_density(f, N, uu);
+  }
+  default:
+    break;
   }
   rho_u(f, &rho, u);
   return rho;
@@ -760,10 +940,30 @@ static real momenta_rho(const lattice_t* L, uint64_t n)
 static void momenta_u(const lattice_t* L, uint64_t n, real u[3])
 {
   const dyn_t* D = &L->table[L->mask[n]];
-  if (D->kind == KIND_LOCAL_VELOCITY || D->kind == KIND_FIXED_VELOCITY) {
+  switch (D->kind) {
+  case KIND_NONE:
+  case KIND_BOUNCE_BACK:
+    for (int a = 0; a < 3; ++a) u[a] = 0;   /* ZeroMomentum */
+    return;
+  case KIND_LOCAL_VELOCITY:
+  case KIND_FIXED_VELOCITY:
+  case KIND_INNER_EDGE:
+  case KIND_INNER_CORNER:
+  case KIND_BOUNCE_BACK_VELOCITY: {
     const real* uw = wall_velocity(L, D, n);
     for (int a = 0; a < 3; ++a) u[a] = uw[a];
     return;
+  }
+  case KIND_LOCAL_PRESSURE: {
+    real f[Q];
+    int dir = 0, orient = 0;
+    load_cell(L, n, f);
+    flat_normal(L, n, &dir, &orient);
+    pressure_u(f, dir, orient, wall_density(L, D, n), wall_velocity(L, D, n), u);
+    return;
+  }
+  default:
+    break;
   }
   real f[Q], rho;
   load_cell(L, n, f);
@@ -933,6 +1133,13 @@ void ORC(set_bouzidi)(void* h, int64_t n, const int64_t* cells, const real* q)
   L->bz_n = n;
 }
 
+/* BlockLattice::postProcess() (core/blockLattice.hh:56-67): the PostStream
+ * processors on the current populations, as initialize() runs them once */
+void ORC(post_process)(void* h)
+{
+  post_stream((lattice_t*)h);
+}
+
 void ORC(collide_and_stream)(void* h, int steps)
 {
   lattice_t* L = (lattice_t*)h;
@@ -940,6 +1147,9 @@ void ORC(collide_and_stream)(void* h, int steps)
     collide(L);
     periodic_copy(L);
     stream(L);
+    if (L->bz_n > 0 || L->normals) {
+      periodic_refresh(L);
+    }
     post_stream(L);
     L->time += 1;
   }

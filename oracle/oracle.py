@@ -51,6 +51,7 @@ def library() -> ctypes.CDLL:
             g("set_cell_force").argtypes = [vp, vp]
             g("set_cell_normals").argtypes = [vp, vp]
             g("set_bouzidi").argtypes = [vp, ctypes.c_int64, vp, vp]
+            g("post_process").argtypes = [vp]
             g("get_populations").argtypes = [vp, vp]
             g("set_populations").argtypes = [vp, vp]
             g("collide_and_stream").argtypes = [vp, i32]
@@ -158,6 +159,10 @@ class OracleLattice:
         v = np.ascontiguousarray(normals, dtype=np.int8).reshape(-1)
         assert v.size == 3 * self.ncells
         self._f("set_cell_normals")(self._h, _ptr(v))
+
+    def post_process(self, stream=None) -> None:
+        """the PostStream processors on the current populations"""
+        self._f("post_process")(self._h)
 
     def set_bouzidi(self, cells, q) -> None:
         """the cells running BouzidiPostProcessor (applied in the given order)

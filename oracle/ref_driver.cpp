@@ -22,7 +22,8 @@
  *   olb_ref poiseuille N OUT                      forced BGK + BounceBack pipe, run to
  *                                                convergence (the reference's KAT)
  *   olb_ref poiseuille_bouzidi N OUT              the same with the Bouzidi wall (its bouzidi row)
- *   olb_ref poiseuille_local N OUT                its forced local row (LocalVelocity wall)
+ *   olb_ref poiseuille_local N OUT [STEPS]        its forced local row (LocalVelocity wall); STEPS
+ *                                                stops after that many steps instead
  *   olb_ref poiseuille_local_nonforced N OUT      its non-forced local row (LocalVelocity wall and
  *                                                inflow, LocalPressure outflow)
  *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
@@ -451,7 +452,7 @@ int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir
 // and energies, the discrete normals of materials 2-4, every cell's
 // computeRhoU after initialize (the velocity and density the boundary momenta
 // hold) and the analytic velocity and pressure of the fluid.
-int run_poiseuille_local(int n, const std::string& out, bool forced)
+int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSteps = -1)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
   const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
@@ -558,7 +559,7 @@ int run_poiseuille_local(int n, const std::string& out, bool forced)
   std::vector<double> energy;
   int iT;
   for (iT = 0; iT < (int)converter.getLatticeTime(maxPhysT); ++iT) {
-    if (converge.hasConverged()) {
+    if (fixedSteps >= 0 ? iT == fixedSteps : converge.hasConverged()) {
       break;
     }
     L.collideAndStream();
@@ -797,8 +798,9 @@ int main(int argc, char** argv)
     if (mode == "poiseuille" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3]);
     }
-    if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && argc == 4) {
-      return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local");
+    if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && (argc == 4 || argc == 5)) {
+      return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local",
+                                  argc == 5 ? std::stoi(argv[4]) : -1);
     }
     if (mode == "poiseuille_bouzidi" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3], true);

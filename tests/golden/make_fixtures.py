@@ -233,6 +233,8 @@ def main() -> None:
                 poiseuille_bouzidi()
             elif name == "vtk":
                 vtk_fixtures()
+            elif name == "poiseuille_local":
+                poiseuille_local()
             else:
                 full_size(name)
         return
@@ -308,6 +310,47 @@ def vtk_fixtures() -> None:
         with open(os.path.join(vtk, f"{name}.pvd")) as fh:
             texts["pvd"] = fh.read()
         save(fixture, meta, arrays, extra={"vtk_name": name}, texts=texts)
+
+
+def poiseuille_local() -> None:
+    """testPoiseuille3DBGK's local rows at N = 11, FP64, run to convergence by
+    the reference: forced (LocalVelocity wall, periodic) and non-forced
+    (LocalVelocity wall and inflow, LocalPressure outflow)"""
+    for mode, name in (("poiseuille_local", "poiseuille11_forced_local_f64"),
+                       ("poiseuille_local_nonforced", "poiseuille11_nonforced_local_f64")):
+        path = os.path.join(RUN, mode + ".bin")
+        run("olb_ref_d", [mode, "11", path])
+        meta, arrays = read_dump(path)
+        mats = arrays["materials"]
+        keep = np.flatnonzero(mats != 0)
+        fluid = np.flatnonzero(mats == 1)
+        q = 19
+        n = arrays["f0"].size // q
+        p = meta["padding"]
+        shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+        core = np.zeros(shape, dtype=bool)
+        core[p:-p, p:-p, p:-p] = True
+        wall = np.flatnonzero(core.reshape(-1) & (mats >= 2))
+        out = {
+            "meta": np.array(json.dumps(meta)),
+            "materials": mats.astype(np.uint8),
+            "distances": arrays["distances"],
+            "cells": keep.astype(np.int64),
+            "f0": arrays["f0"].reshape(q, n)[:, keep],
+            "fluid": fluid.astype(np.int64),
+            "f1_fluid": arrays["f1"].reshape(q, n)[:, fluid],
+            "wall": wall.astype(np.int64),
+            "f1_wall": arrays["f1"].reshape(q, n)[:, wall],
+            "energy": arrays["energy"],
+            "usol_fluid": arrays["usol"].reshape(n, 3)[fluid],
+            "psol_fluid": arrays["psol"][fluid],
+            "rho0": arrays["rho0"][keep],
+            "u0": arrays["u0"].reshape(n, 3)[keep],
+            "normal_type": np.asarray(arrays["normal_type"]).astype(np.int8),
+            "normals": np.asarray(arrays["normals"]).astype(np.int8),
+        }
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print("wrote", name, meta)
 
 
 def poiseuille_bouzidi() -> None:

@@ -7,9 +7,11 @@ import os
 
 import numpy as np
 
-from openlb_amd import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FORCED_BGK, FORCED_TRT, NO_DYNAMICS,
-                        SMAGORINSKY_BGK, Dynamics)
-from openlb_amd.cases import EXTERNAL_CORNER, EXTERNAL_EDGE, FLAT, LocalVelocity
+from openlb_amd import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_RHO, FIELD_VELOCITY, FIXED_VELOCITY,
+                        FORCED_BGK, FORCED_TRT, INNER_CORNER_VELOCITY, INNER_EDGE_VELOCITY, LOCAL_PRESSURE,
+                        LOCAL_VELOCITY, NO_DYNAMICS, SMAGORINSKY_BGK, Dynamics)
+from openlb_amd.cases import (EXTERNAL_CORNER, EXTERNAL_EDGE, FLAT, INTERNAL_CORNER, INTERNAL_EDGE, LocalPressure,
+                              LocalVelocity)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -56,14 +58,24 @@ def dynamics_for(meta: dict, storage: str) -> dict:
                             force=(_as_value(meta["fx"], storage), 0.0, 0.0)),
                 2: Dynamics(BOUNCE_BACK)}
     if case == "poiseuille":
+        forced = Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0))
+        if meta.get("boundary") == "local":
+            # testPoiseuille3DBGK.cpp:118-199: the bulk dynamics on materials 1-4,
+            # then LocalVelocity (2, 3) and LocalPressure (4) on top
+            bulk = forced if meta["forced"] else Dynamics(BGK, omega=meta["omega"])
+            if meta["forced"]:
+                return {1: bulk, 2: LocalVelocity(omega=meta["omega"], base=bulk)}
+            return {1: bulk, 2: LocalVelocity(omega=meta["omega"], base=bulk),
+                    3: LocalVelocity(omega=meta["omega"], base=bulk),
+                    4: LocalPressure(omega=meta["omega"], base=bulk)}
         wall = Dynamics(NO_DYNAMICS) if meta.get("boundary") == "bouzidi" else Dynamics(BOUNCE_BACK)
-        return {1: Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0)), 2: wall}
+        return {1: forced, 2: wall}
     raise ValueError(case)
 
 
 def periodic_for(meta: dict):
     return {"cavity": (False,) * 3, "cavity_lv": (False,) * 3, "tgv": (True,) * 3, "porous": (True,) * 3,
-            "poiseuille": (True, False, False)}[meta["case"]]
+            "poiseuille": (bool(meta.get("forced", 1)), False, False)}[meta["case"]]
 
 
 def build(lattice_cls, meta, d, storage: str):
@@ -77,13 +89,29 @@ def build(lattice_cls, meta, d, storage: str):
     slot = 1
     lv = {}
     for mat, dyn in sorted(dyns.items()):
-        if isinstance(dyn, LocalVelocity):
-            from openlb_amd import FIXED_VELOCITY, LOCAL_VELOCITY
-
-            lat.set_dynamics(slot, Dynamics(LOCAL_VELOCITY, omega=dyn.omega, wall_velocity=dyn.wall_velocity))
-            lat.set_dynamics(slot + 1, Dynamics(FIXED_VELOCITY, omega=dyn.omega, wall_velocity=dyn.wall_velocity))
-            lv[mat] = (slot, slot + 1)
-            slot += 2
+        if isinstance(dyn, (LocalVelocity, LocalPressure)):
+            # one slot per boundary shape; the material's base dynamics for
+            # cells boundary::set gave no normal
+            if isinstance(dyn, LocalVelocity):
+                shapes = {FLAT: LOCAL_VELOCITY, EXTERNAL_EDGE: FIXED_VELOCITY, EXTERNAL_CORNER: FIXED_VELOCITY,
+                          INTERNAL_EDGE: INNER_EDGE_VELOCITY, INTERNAL_CORNER: INNER_CORNER_VELOCITY}
+                extra = {"wall_velocity": dyn.wall_velocity}
+            else:
+                shapes = {FLAT: LOCAL_PRESSURE}
+                extra = {"wall_rho": dyn.rho}
+            by_type = {}
+            kinds = {}
+            for t, kind in shapes.items():
+                if kind not in kinds:
+                    lat.set_dynamics(slot, Dynamics(kind, omega=dyn.omega, **extra))
+                    kinds[kind] = slot
+                    slot += 1
+                by_type[t] = kinds[kind]
+            if dyn.base is not None:
+                lat.set_dynamics(slot, dyn.base)
+                lut[mat] = slot
+                slot += 1
+            lv[mat] = by_type
         else:
             lat.set_dynamics(slot, dyn)
             lut[mat] = slot
@@ -92,12 +120,20 @@ def build(lattice_cls, meta, d, storage: str):
     if lv:
         # the reference's own discrete normals of the boundary cells
         types = d["normal_type"]
-        for mat, (flat, fixed) in lv.items():
+        for mat, by_type in lv.items():
             on = mats == mat
-            mask[on & (types == FLAT)] = flat
-            mask[on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))] = fixed
+            for t, sl in by_type.items():
+                mask[on & (types == t)] = sl
         lat.set_cell_normals(d["normals"])
     lat.set_dynamics_mask(mask)
+    if "u0" in d:
+        # the velocity and density the boundary momenta hold (computeRhoU at t = 0)
+        u = np.zeros((lat.ncells, 3))
+        u[d["cells"]] = d["u0"]
+        rho = np.ones(lat.ncells)
+        rho[d["cells"]] = d["rho0"]
+        lat.set_cell_field(FIELD_VELOCITY, u)
+        lat.set_cell_field(FIELD_RHO, rho)
     if "field_cells" in d:
         # the reference's own BOUZIDI_DISTANCE field, solid-solid pairs resolved
         lat.set_bouzidi(*bouzidi_entries(meta, d))
