@@ -17,9 +17,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT, LOCAL_PRESSURE,
-               LOCAL_VELOCITY,
-               NO_DYNAMICS, Dynamics)
+from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT,
+               INNER_CORNER_VELOCITY, INNER_EDGE_VELOCITY, LOCAL_PRESSURE, LOCAL_VELOCITY, NO_DYNAMICS, Dynamics)
+from .normals import EXTERNAL_CORNER, EXTERNAL_EDGE, FLAT, INTERNAL_CORNER, INTERNAL_EDGE, discrete_normals
 
 __all__ = [
     "Case",
@@ -65,23 +65,41 @@ class Case:
 
     def dynamics_slots(self):
         """material -> dynamics table slot (slot 0 stays NoDynamics).  A
-        LocalVelocity material takes two slots, keyed (material, "flat") for
-        its faces and (material, "fixed") for its edges and corners."""
+        LocalVelocity or LocalPressure material takes one slot per boundary
+        shape its cells have, keyed (material, discrete normal type), and one
+        for its base dynamics (keyed by the material) if it has one."""
         slots = {}
         table = [Dynamics(NO_DYNAMICS)]
+        normals = self.cell_normals()
         for mat in sorted(self.dynamics):
             d = self.dynamics[mat]
-            if isinstance(d, LocalVelocity):
-                table.append(Dynamics(LOCAL_VELOCITY, omega=d.omega, wall_velocity=d.wall_velocity))
-                slots[(mat, "flat")] = len(table) - 1
-                table.append(Dynamics(FIXED_VELOCITY, omega=d.omega, wall_velocity=d.wall_velocity))
-                slots[(mat, "fixed")] = len(table) - 1
-            elif isinstance(d, LocalPressure):
-                table.append(Dynamics(LOCAL_PRESSURE, omega=d.omega, wall_rho=d.rho))
-                slots[(mat, "flat")] = len(table) - 1
+            if isinstance(d, (LocalVelocity, LocalPressure)):
+                on = self.materials == mat
+                types = np.where(normals[1][on].any(axis=-1) | (normals[0][on] != FLAT), normals[0][on], -1)
+                if isinstance(d, LocalVelocity):
+                    shapes = ((FLAT, LOCAL_VELOCITY), (EXTERNAL_EDGE, FIXED_VELOCITY),
+                              (EXTERNAL_CORNER, FIXED_VELOCITY), (INTERNAL_EDGE, INNER_EDGE_VELOCITY),
+                              (INTERNAL_CORNER, INNER_CORNER_VELOCITY))
+                    extra = {"wall_velocity": d.wall_velocity}
+                else:
+                    shapes = ((FLAT, LOCAL_PRESSURE),)
+                    extra = {"wall_rho": d.rho}
+                by_kind = {}
+                for t, kind in shapes:
+                    if not np.any(types == t):
+                        continue
+                    if kind not in by_kind:
+                        table.append(Dynamics(kind, omega=d.omega, **extra))
+                        by_kind[kind] = len(table) - 1
+                    slots[(mat, t)] = by_kind[kind]
+                if d.base is not None:
+                    table.append(d.base)
+                    slots[mat] = len(table) - 1
             else:
                 table.append(d)
                 slots[mat] = len(table) - 1
+        if len(table) > 16:
+            raise ValueError(f"{len(table)} dynamics slots needed; the lattice holds 16")
         return slots, table
 
     def local_velocity_materials(self) -> tuple:
@@ -106,22 +124,24 @@ class Case:
         mask = lut[self.materials.reshape(-1)]
         normals = self.cell_normals()
         if normals is not None:
+            # boundary::set (boundary/setBoundary.h:76-100): a classified cell
+            # gets the dynamics of its shape; a cell with no discrete normal
+            # (FLAT with the zero normal) keeps the material's base dynamics
+            # under LocalVelocity (boundary/localVelocity3D.h:47-82) and is an
+            # error under LocalPressure (boundary/localPressure3D.h:47-58)
             types = normals[0].reshape(-1)
+            unset = ~normals[1].reshape(-1, 3).any(axis=1)
             mats = self.materials.reshape(-1)
             for m in self.local_velocity_materials():
-                on = mats == m
-                flat = on & (types == FLAT)
-                if isinstance(self.dynamics[m], LocalPressure):
-                    if np.any(on & ~flat):
-                        raise ValueError("LocalPressure takes flat faces only")
-                    mask[flat] = slots[(m, "flat")]
-                    continue
-                fixed = on & ((types == EXTERNAL_EDGE) | (types == EXTERNAL_CORNER))
-                if np.any(on & ~flat & ~fixed):
-                    raise ValueError("the box classifier covers flat faces, external edges and external "
-                                     "corners; this geometry has other boundary cells")
-                mask[flat] = slots[(m, "flat")]
-                mask[fixed] = slots[(m, "fixed")]
+                on = (mats == m) & (types >= 0)
+                if isinstance(self.dynamics[m], LocalPressure) and np.any(on & ((types != FLAT) | unset)):
+                    raise ValueError("No valid discrete normal found. This BC is not suited for curved walls.")
+                for t in (FLAT, EXTERNAL_EDGE, EXTERNAL_CORNER, INTERNAL_EDGE, INTERNAL_CORNER):
+                    if (m, t) in slots:
+                        sel = on & (types == t)
+                        if t == FLAT:
+                            sel &= ~unset
+                        mask[sel] = slots[(m, t)]
         return mask
 
     def apply(self, lattice) -> None:
@@ -175,74 +195,6 @@ class LocalPressure:
     omega: float
     rho: float = 1.0
     base: object = None
-
-
-# DiscreteNormalType (geometry/discreteNormals.h:33-39)
-FLAT, EXTERNAL_CORNER, INTERNAL_CORNER, EXTERNAL_EDGE, INTERNAL_EDGE = 0, 1, 2, 3, 4
-
-
-def discrete_normals(materials: np.ndarray, boundary, fluid: int = 1, outside: int = 0):
-    """Type and outward discrete normal of every cell whose material is in
-    `boundary`, by the neighbourhood rules of computeBoundaryTypeAndNormal
-    (geometry/discreteNormals.hh:176-1190) for the configurations found on
-    box walls; B means neither fluid nor outside:
-      flat (FLAT): the four in-plane axis neighbours are B and one neighbour
-        along the remaining axis is fluid; the normal points away from it;
-      external edge (EXTERNAL_EDGE), along axis p: both neighbours along p are
-        B, the transverse neighbours s_a e_a and s_b e_b are B, the cells
-        +-e_p + s_b e_b are not fluid and the diagonal s_a e_a + s_b e_b is
-        fluid; normal -(s_a e_a + s_b e_b);
-      external corner (EXTERNAL_CORNER): for signs s, the six cells s_x e_x,
-        s_y e_y, s_z e_z and their pairwise sums are B and s is fluid; normal -s.
-    Cells matching none or several of these (inner edges and corners) get
-    type -2.  Returns (type per cell, -1 off the boundary; normals, int8 x 3)."""
-    m = np.asarray(materials)
-    shape = m.shape
-    P = np.pad(m, 2, constant_values=outside)
-    isf = P == fluid
-    isb = (P != fluid) & (P != outside)
-    types = np.full(shape, -1, dtype=np.int8)
-    normals = np.zeros(shape + (3,), dtype=np.int8)
-    cells = np.argwhere(np.isin(m, list(boundary)))
-
-    def at(arr, c, d):
-        return arr[c[0] + 2 + d[0], c[1] + 2 + d[1], c[2] + 2 + d[2]]
-
-    E = np.eye(3, dtype=int)
-    for c in cells:
-        found = []
-        for a in range(3):
-            o1, o2 = E[(a + 1) % 3], E[(a + 2) % 3]
-            if all(at(isb, c, v) for v in (o1, -o1, o2, -o2)):
-                for s in (1, -1):
-                    if at(isf, c, s * E[a]):
-                        found.append((FLAT, -s * E[a]))
-        for p in range(3):
-            a_ax, b_ax = (p + 1) % 3, (p + 2) % 3
-            if not (at(isb, c, E[p]) and at(isb, c, -E[p])):
-                continue
-            for sa in (1, -1):
-                for sb in (1, -1):
-                    for ea, eb in ((sa * E[a_ax], sb * E[b_ax]), (sb * E[b_ax], sa * E[a_ax])):
-                        if (at(isb, c, eb) and not at(isf, c, E[p] + eb) and not at(isf, c, -E[p] + eb)
-                                and at(isb, c, ea) and at(isf, c, ea + eb)):
-                            found.append((EXTERNAL_EDGE, -(ea + eb)))
-        for sx in (1, -1):
-            for sy in (1, -1):
-                for sz in (1, -1):
-                    s3 = np.array([sx, sy, sz])
-                    ex, ey, ez = sx * E[0], sy * E[1], sz * E[2]
-                    if (all(at(isb, c, v) for v in (ex, ey, ez, ex + ey, ex + ez, ey + ez))
-                            and at(isf, c, s3)):
-                        found.append((EXTERNAL_CORNER, -s3))
-        uniq = {(t, tuple(int(v) for v in n)) for t, n in found}
-        if len(uniq) == 1:
-            t, n = uniq.pop()
-            types[tuple(c)] = t
-            normals[tuple(c)] = n
-        else:
-            types[tuple(c)] = -2
-    return types, normals
 
 
 def _padded(extent, padding):
@@ -470,6 +422,45 @@ def cavity3d_local_velocity(res: int, u_lid: float = 0.1, re: float = 100.0, pad
     c.name = f"cavity3d_lv_res{res}"
     c.dynamics[2] = LocalVelocity(omega=omega)
     c.dynamics[3] = LocalVelocity(omega=omega, wall_velocity=(u_lid, 0.0, 0.0))
+    return c
+
+
+def step_cavity(res: int, u_lid: float = 0.1, re: float = 100.0, step: int | None = None, padding: int = 3,
+                dtype=np.float64) -> Case:
+    """The LocalVelocity cavity of cavity3d_local_velocity with a solid cube
+    in the corner at the origin: core cells with x, y, z all <= `step`
+    (default res // 2) are solid, wall (material 2) where a fluid cell is
+    among their 26 neighbours and outside (0) elsewhere.  The cube's three
+    edges inside the fluid are internal edges, its tip an internal corner,
+    so boundary::set gives the walls every LocalVelocity shape.  Wall cells
+    without a discrete normal are BounceBack."""
+    c = cavity3d_local_velocity(res, u_lid=u_lid, re=re, padding=padding, dtype=dtype)
+    s = res // 2 if step is None else int(step)
+    if not 1 <= s <= res - 3:
+        raise ValueError("the step must leave fluid on every side")
+    p = padding
+    core = c.materials[p:-p, p:-p, p:-p]
+    fluid = core == 1
+    fluid[: s + 1, : s + 1, : s + 1] = False
+    block = np.zeros_like(fluid)
+    block[: s + 1, : s + 1, : s + 1] = True
+    near = np.zeros_like(fluid)
+    padded = np.pad(fluid, 1)
+    n = core.shape
+    for dx in (0, 1, 2):
+        for dy in (0, 1, 2):
+            for dz in (0, 1, 2):
+                near |= padded[dx:dx + n[0], dy:dy + n[1], dz:dz + n[2]]
+    inner = block.copy()
+    inner[0, :, :] = inner[:, 0, :] = inner[:, :, 0] = False
+    core[inner & near] = 2
+    core[inner & ~near] = 0
+    c.name = f"step_cavity_res{res}_step{s}"
+    omega = c.info["omega"]
+    c.dynamics[2] = LocalVelocity(omega=omega, base=Dynamics(BOUNCE_BACK))
+    c.dynamics[3] = LocalVelocity(omega=omega, wall_velocity=(u_lid, 0.0, 0.0), base=Dynamics(BOUNCE_BACK))
+    c.info["step"] = s
+    c.info.pop("_normals", None)
     return c
 
 

@@ -9,13 +9,16 @@
  *   cavity3d [--RESOLUTION n] [--TIME_STEPS k] [--LID_VELOCITY u] [--REYNOLDS re]
  *            [--STORAGE f32|f16|bf16] [--DUMP file] [--TUNE_CACHE file]
  *            [--BOUNDARY local_velocity|bounce_back] [--SAVE file] [--LOAD file]
- *            [--VTK dir]
+ *            [--VTK dir] [--STEP s]
  *
  * --DUMP writes the materials and the populations of the padded block after
  * the run (logical order, float32) for the tests.  --VTK writes velocity,
  * density, physVelocity and physPressure after the run with SuperVTMwriter3D
  * into dir (cuboid of length 1, spacing 1/RESOLUTION, lattice-unit factors),
- * as the reference's getResults does with its own writer.
+ * as the reference's getResults does with its own writer.  --STEP s makes
+ * the core cells with x, y, z <= s a solid cube (wall material 2 next to the
+ * fluid, outside elsewhere), so the LocalVelocity walls take every boundary
+ * shape; wall cells without a discrete normal stay BounceBack.
  */
 #include <chrono>
 #include <cstdio>
@@ -43,6 +46,7 @@ struct Parameters {
   std::string save;           // checkpoint after the timed steps
   std::string load;           // checkpoint to restart from before the warm-up
   std::string vtk;            // VTK output directory
+  int step = -1;              // solid corner cube x, y, z <= step (-1: none)
 };
 
 static Parameters fromCLI(int argc, char** argv)
@@ -59,6 +63,7 @@ static Parameters fromCLI(int argc, char** argv)
     else if (k == "--SAVE") p.save = v;
     else if (k == "--LOAD") p.load = v;
     else if (k == "--VTK") p.vtk = v;
+    else if (k == "--STEP") p.step = std::stoi(v);
     else if (k == "--BOUNDARY") {
       if (v != "local_velocity" && v != "bounce_back") {
         throw std::invalid_argument("--BOUNDARY is local_velocity or bounce_back");
@@ -76,13 +81,32 @@ static Parameters fromCLI(int argc, char** argv)
 
 // the cavity3dBenchmark geometry at RESOLUTION n: (n+1)^3 nodes, walls
 // (material 2), the lid face y = n without its rim (material 3), fluid (1)
-static void prepareGeometry(SuperGeometry<T>& geometry, int n)
+static void prepareGeometry(SuperGeometry<T>& geometry, int n, int step)
 {
   geometry.rename(0, 2);
   geometry.rename(2, 1, {1, 1, 1});
   geometry.rename(2, 3, [n](int x, int y, int z) {
     return y == n && x > 0 && x < n && z > 0 && z < n;
   });
+  if (step < 0) {
+    return;
+  }
+  if (step < 1 || step > n - 3) {
+    throw std::invalid_argument("--STEP must leave fluid on every side of the cube");
+  }
+  auto inCube = [step](int x, int y, int z) { return x <= step && y <= step && z <= step; };
+  geometry.rename(1, 2, [&](int x, int y, int z) { return inCube(x, y, z); });
+  // cube cells with no fluid among their 26 neighbours are outside
+  std::vector<std::array<int, 3>> outside;
+  geometry.forCore([&](int x, int y, int z) {
+    if (!inCube(x, y, z) || x == 0 || y == 0 || z == 0) return;
+    for (int i = -1; i <= 1; ++i)
+      for (int j = -1; j <= 1; ++j)
+        for (int k = -1; k <= 1; ++k)
+          if (geometry.getOrOutside(x + i, y + j, z + k) == 1) return;
+    outside.push_back({x, y, z});
+  });
+  for (const auto& c : outside) geometry.set(c[0], c[1], c[2], 0);
 }
 
 static void prepareLattice(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry,
@@ -90,6 +114,10 @@ static void prepareLattice(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T
 {
   dynamics::set<BGKdynamics>(lattice, geometry, 1);
   if (p.localVelocity) {
+    if (p.step >= 0) {
+      boundary::set<boundary::BounceBack>(lattice, geometry, 2);
+      boundary::set<boundary::BounceBack>(lattice, geometry, 3);
+    }
     boundary::set<boundary::LocalVelocity>(lattice, geometry, 2);
     boundary::set<boundary::LocalVelocity>(lattice, geometry, 3);
   } else {
@@ -114,7 +142,7 @@ int main(int argc, char** argv)
     const Parameters p = fromCLI(argc, argv);
     const int n = p.resolution;
     SuperGeometry<T> geometry({n + 1, n + 1, n + 1}, 3);
-    prepareGeometry(geometry, n);
+    prepareGeometry(geometry, n, p.step);
     SuperLattice<T, DESCRIPTOR> lattice(geometry, p.storage);
     prepareLattice(lattice, geometry, p);
     setInitialValues(lattice, geometry, p);

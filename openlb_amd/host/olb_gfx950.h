@@ -235,6 +235,18 @@ public:
         for (int z = 0; z < _extent[2]; ++z) f(x, y, z);
   }
 
+  /// every cell of the padded block at least `radius` cells from its edge
+  /// (forSpatialLocations with getNeighborhoodRadius,
+  /// core/blockStructure.h:223-250)
+  template <typename F>
+  void forSpatialLocations(int radius, F f) const
+  {
+    const int lo = radius - _overlap;
+    for (int x = lo; x < _extent[0] + _overlap - radius; ++x)
+      for (int y = lo; y < _extent[1] + _overlap - radius; ++y)
+        for (int z = lo; z < _extent[2] + _overlap - radius; ++z) f(x, y, z);
+  }
+
 private:
   int getClamped(int x, int y, int z) const
   {
@@ -312,76 +324,107 @@ inline std::vector<std::uint8_t> unbase64(const std::string& text)
 }
 
 // ---- discrete normals (geometry/discreteNormals.h:33-57) ------------------
-enum class NormalType { None, Flat, ExternalEdge, ExternalCorner };
+/// DiscreteNormalType, with the reference's values
+enum class NormalType : int { Flat = 0, ExternalCorner = 1, InternalCorner = 2, ExternalEdge = 3, InternalEdge = 4 };
 
+struct NormalPattern {
+  NormalType type;
+  std::array<int, 3> normal;
+  const char* tests;  // "Kxyz" words: K in F (fluid), B (neither fluid nor outside), N (not fluid)
+};
 
-/// type and outward normal of a boundary cell by the neighbourhood rules
-/// of computeBoundaryTypeAndNormal (geometry/discreteNormals.hh:176-1190)
-/// for box walls; B = neither fluid (1) nor outside (0):
-///  flat: the four in-plane axis neighbours are B, one neighbour along the
-///   remaining axis is fluid, the normal points away from it;
-///  external edge along p: neighbours +-e_p are B, transverse s_a e_a and
-///   s_b e_b are B, +-e_p + s_b e_b are not fluid, s_a e_a + s_b e_b is
-///   fluid; normal -(s_a e_a + s_b e_b);
-///  external corner: s_x e_x, s_y e_y, s_z e_z and their pairwise sums are
-///   B and s is fluid; normal -s.
-/// A cell that matches none or several gives None.
+/// the neighbourhood patterns of computeBoundaryTypeAndNormal
+/// (geometry/discreteNormals.hh:176-1190) in the reference's order; every
+/// pattern also needs the cell itself to be B.  The first flat, corner or
+/// external-edge pattern that matches decides; an internal-edge pattern that
+/// matches afterwards replaces it (the last one wins).
+inline const std::vector<NormalPattern>& normalPatterns()
+{
+  static const std::vector<NormalPattern> patterns = {
+      {NormalType::Flat, {-1, 0, 0}, "F+00 B0+0 B0-0 B00+ B00-"},
+      {NormalType::Flat, {1, 0, 0}, "F-00 B0+0 B0-0 B00+ B00-"},
+      {NormalType::Flat, {0, -1, 0}, "F0+0 B+00 B-00 B00+ B00-"},
+      {NormalType::Flat, {0, 1, 0}, "F0-0 B+00 B-00 B00+ B00-"},
+      {NormalType::Flat, {0, 0, -1}, "F00+ B+00 B-00 B0+0 B0-0"},
+      {NormalType::Flat, {0, 0, 1}, "F00- B+00 B-00 B0+0 B0-0"},
+      {NormalType::ExternalCorner, {-1, -1, -1}, "F+++ B++0 B+0+ B+00 B0++ B0+0 B00+"},
+      {NormalType::ExternalCorner, {-1, 1, -1}, "F+-+ B+-0 B+0+ B+00 B0-+ B0-0 B00+"},
+      {NormalType::ExternalCorner, {-1, 1, 1}, "F+-- B+-0 B+0- B+00 B0-- B0-0 B00-"},
+      {NormalType::ExternalCorner, {-1, -1, 1}, "F++- B++0 B+0- B+00 B0+- B0+0 B00-"},
+      {NormalType::ExternalCorner, {1, 1, 1}, "F--- B--0 B-0- B-00 B0-- B0-0 B00-"},
+      {NormalType::ExternalCorner, {1, -1, 1}, "F-+- B-+0 B-0- B-00 B0+- B0+0 B00-"},
+      {NormalType::ExternalCorner, {1, -1, -1}, "F-++ B-+0 B-0+ B-00 B0++ B0+0 B00+"},
+      {NormalType::ExternalCorner, {1, 1, -1}, "F--+ B--0 B-0+ B-00 B0-+ B0-0 B00+"},
+      {NormalType::InternalCorner, {1, 1, 1}, "F-00 F0-0 F00- B+00 B0+0 B00+"},
+      {NormalType::InternalCorner, {1, -1, 1}, "F-00 F0+0 F00- B+00 B0-0 B00+"},
+      {NormalType::InternalCorner, {1, -1, -1}, "F-00 F0+0 F00+ B+00 B0-0 B00-"},
+      {NormalType::InternalCorner, {1, 1, -1}, "F-00 F0-0 F00+ B+00 B0+0 B00-"},
+      {NormalType::InternalCorner, {-1, 1, 1}, "F+00 F0-0 F00- B-00 B0+0 B00+"},
+      {NormalType::InternalCorner, {-1, -1, 1}, "F+00 F0+0 F00- B-00 B0-0 B00+"},
+      {NormalType::InternalCorner, {-1, 1, -1}, "F+00 F0-0 F00+ B-00 B0+0 B00-"},
+      {NormalType::InternalCorner, {-1, -1, -1}, "F+00 F0+0 F00+ B-00 B0-0 B00-"},
+      {NormalType::ExternalEdge, {0, 1, -1}, "F0-+ B+00 B-00 B0-0 B00+ N+0+ N-0+"},
+      {NormalType::ExternalEdge, {0, -1, -1}, "F0++ B+00 B-00 B0+0 B00+ N+0+ N-0+"},
+      {NormalType::ExternalEdge, {0, -1, 1}, "F0+- B+00 B-00 B00- N+0- N-0- N0+0"},
+      {NormalType::ExternalEdge, {0, 1, 1}, "F0-- B+00 B-00 B00- N+0- N-0- N0-0"},
+      {NormalType::ExternalEdge, {-1, 0, -1}, "F+0+ B+00 B0+0 B0-0 B00+ N-00 N00-"},
+      {NormalType::ExternalEdge, {1, 0, -1}, "F-0+ B-00 B0+0 B0-0 B00+ N+00 N00-"},
+      {NormalType::ExternalEdge, {-1, 0, 1}, "F+0- B+00 B0+0 B0-0 B00- N-00 N00+"},
+      {NormalType::ExternalEdge, {1, 0, 1}, "F-0- B-00 B0+0 B0-0 B00- N+00 N00+"},
+      {NormalType::ExternalEdge, {-1, -1, 0}, "F++0 B0+0 B00+ B00- N+00"},
+      {NormalType::ExternalEdge, {1, -1, 0}, "F-+0 B0+0 B00+ B00- N-00"},
+      {NormalType::ExternalEdge, {1, 1, 0}, "F--0 B0-0 B00+ B00- N-00"},
+      {NormalType::ExternalEdge, {-1, 1, 0}, "F+-0 B0-0 B00+ B00- N+00"},
+      {NormalType::InternalEdge, {0, -1, -1}, "F0+0 F00+ B+-0 B+00 B--0 B-00 B0-0 B00-"},
+      {NormalType::InternalEdge, {0, 1, -1}, "F0-0 F00+ B++0 B+00 B-+0 B-00 B0+0 B00-"},
+      {NormalType::InternalEdge, {0, -1, 1}, "F0+0 F00- B+-0 B+00 B--0 B-00 B0-0 B00+"},
+      {NormalType::InternalEdge, {0, 1, 1}, "F0-0 F00- B++0 B+00 B-+0 B-00 B0+0 B00+"},
+      {NormalType::InternalEdge, {1, 0, 1}, "F-00 F00- B+00 B0++ B0+0 B0-+ B0-0 B00+"},
+      {NormalType::InternalEdge, {1, 0, -1}, "F-00 F00+ B+00 B0+- B0+0 B0-- B0-0 B00-"},
+      {NormalType::InternalEdge, {-1, 0, 1}, "F+00 F00- B-00 B0++ B0+0 B0-+ B0-0 B00+"},
+      {NormalType::InternalEdge, {-1, 0, -1}, "F+00 F00+ B-00 B0+- B0+0 B0-- B0-0 B00-"},
+      {NormalType::InternalEdge, {1, 1, 0}, "F--0 F-00 F0-0 B0+0 B00+ B00-"},
+      {NormalType::InternalEdge, {-1, 1, 0}, "F+-0 F+00 F0-0 B0+0 B00+ B00-"},
+      {NormalType::InternalEdge, {1, -1, 0}, "F-+0 F-00 F0+0 B0-0 B00+ B00-"},
+      {NormalType::InternalEdge, {-1, -1, 0}, "F++0 F+00 F0+0 B0-0 B00+ B00-"},
+  };
+  return patterns;
+}
+
+/// type and outward normal of the cell at (x, y, z) (fluid = material 1,
+/// outside = material 0); a cell no pattern matches gives Flat with the
+/// zero normal, as the reference returns it
 template <typename T>
 std::pair<NormalType, std::array<int, 3>> discreteNormal(const SuperGeometry<T>& g, int x, int y, int z)
 {
-  auto m = [&](std::array<int, 3> d) { return g.getOrOutside(x + d[0], y + d[1], z + d[2]); };
-  auto fluid = [&](std::array<int, 3> d) { return m(d) == 1; };
-  auto bnd = [&](std::array<int, 3> d) { const int v = m(d); return v != 1 && v != 0; };
-  auto unit = [](int a, int s) { std::array<int, 3> e{0, 0, 0}; e[a] = s; return e; };
-  auto add = [](std::array<int, 3> u, std::array<int, 3> v) {
-    return std::array<int, 3>{u[0] + v[0], u[1] + v[1], u[2] + v[2]};
+  auto holds = [&](char kind, int dx, int dy, int dz) {
+    const int m = g.getOrOutside(x + dx, y + dy, z + dz);
+    switch (kind) {
+    case 'F': return m == 1;
+    case 'N': return m != 1;
+    default: return m != 1 && m != 0;
+    }
   };
-  std::vector<std::pair<NormalType, std::array<int, 3>>> found;
-  auto note = [&](NormalType t, std::array<int, 3> n) {
-    for (const auto& f : found) {
-      if (f.first == t && f.second == n) return;
+  std::pair<NormalType, std::array<int, 3>> out{NormalType::Flat, {0, 0, 0}};
+  if (!holds('B', 0, 0, 0)) {
+    return out;
+  }
+  auto step = [](char c) { return c == '+' ? 1 : (c == '-' ? -1 : 0); };
+  bool decided = false;
+  for (const NormalPattern& p : normalPatterns()) {
+    if (decided && p.type != NormalType::InternalEdge) {
+      continue;
     }
-    found.emplace_back(t, n);
-  };
-  for (int a = 0; a < 3; ++a) {
-    const int b = (a + 1) % 3, c = (a + 2) % 3;
-    if (bnd(unit(b, 1)) && bnd(unit(b, -1)) && bnd(unit(c, 1)) && bnd(unit(c, -1))) {
-      for (int s : {1, -1}) {
-        if (fluid(unit(a, s))) note(NormalType::Flat, unit(a, -s));
-      }
+    bool match = true;
+    for (const char* w = p.tests; *w && match; w += (w[4] ? 5 : 4)) {
+      match = holds(w[0], step(w[1]), step(w[2]), step(w[3]));
+    }
+    if (match) {
+      out = {p.type, p.normal};
+      decided = true;
     }
   }
-  for (int p = 0; p < 3; ++p) {
-    if (!bnd(unit(p, 1)) || !bnd(unit(p, -1))) continue;
-    const int ax = (p + 1) % 3, bx = (p + 2) % 3;
-    for (int sa : {1, -1}) {
-      for (int sb : {1, -1}) {
-        const std::array<std::array<int, 3>, 2> pairs[2] = {{unit(ax, sa), unit(bx, sb)},
-                                                            {unit(bx, sb), unit(ax, sa)}};
-        for (const auto& pr : pairs) {
-          const auto ea = pr[0], eb = pr[1];
-          if (bnd(eb) && !fluid(add(unit(p, 1), eb)) && !fluid(add(unit(p, -1), eb)) && bnd(ea) &&
-              fluid(add(ea, eb))) {
-            const auto d = add(ea, eb);
-            note(NormalType::ExternalEdge, {-d[0], -d[1], -d[2]});
-          }
-        }
-      }
-    }
-  }
-  for (int sx : {1, -1}) {
-    for (int sy : {1, -1}) {
-      for (int sz : {1, -1}) {
-        const auto ex = unit(0, sx), ey = unit(1, sy), ez = unit(2, sz);
-        if (bnd(ex) && bnd(ey) && bnd(ez) && bnd(add(ex, ey)) && bnd(add(ex, ez)) && bnd(add(ey, ez)) &&
-            fluid({sx, sy, sz})) {
-          note(NormalType::ExternalCorner, {-sx, -sy, -sz});
-        }
-      }
-    }
-  }
-  if (found.size() != 1) return {NormalType::None, {0, 0, 0}};
-  return found[0];
+  return out;
 }
 
 
@@ -435,28 +478,31 @@ public:
   /// LocalVelocity boundary on the cells of `material`
   /// (boundary::set<boundary::LocalVelocity<T,D,BGKdynamics>>,
   /// boundary/setBoundary.h:76-100, boundary/localVelocity3D.h:46-115): each
-  /// cell gets its discrete normal (fluid = material 1, outside = material
-  /// 0); flat faces become regularised velocity cells, external edges and
-  /// corners fixed-velocity BGK cells with their post-processors.  Other
-  /// configurations (internal edges and corners) throw std::domain_error.
+  /// cell at least two cells inside the padded block gets its discrete
+  /// normal (fluid = material 1, outside = material 0); flat faces become
+  /// regularised velocity cells, external edges and corners fixed-velocity
+  /// BGK cells with their post-processors, internal edges and corners the
+  /// inner-edge and inner-corner velocity cells.  A cell without a discrete
+  /// normal keeps the dynamics it had, as in the reference.
   void setLocalVelocity(SuperGeometry<T>& geometry, int material)
   {
-    const int flat = slotFor(OLBGFX_LOCAL_VELOCITY);
-    const int fixed = slotFor(OLBGFX_FIXED_VELOCITY);
     if (_normals.empty()) {
       _normals.assign(3 * _ncells, 0);
     }
-    geometry.forCore([&](int x, int y, int z) {
+    geometry.forSpatialLocations(2, [&](int x, int y, int z) {
       if (geometry.get(x, y, z) != material) return;
       const auto [type, n] = discreteNormal(geometry, x, y, z);
-      const CellID c = geometry.getCellId(x, y, z);
-      if (type == NormalType::Flat) {
-        _slotOfCell[c] = (std::uint8_t)flat;
-      } else if (type == NormalType::ExternalEdge || type == NormalType::ExternalCorner) {
-        _slotOfCell[c] = (std::uint8_t)fixed;
-      } else {
-        throw std::domain_error("LocalVelocity on this platform covers flat faces, external edges and corners");
+      if (n[0] == 0 && n[1] == 0 && n[2] == 0) return;
+      int kind = OLBGFX_LOCAL_VELOCITY;
+      switch (type) {
+      case NormalType::Flat: kind = OLBGFX_LOCAL_VELOCITY; break;
+      case NormalType::ExternalEdge:
+      case NormalType::ExternalCorner: kind = OLBGFX_FIXED_VELOCITY; break;
+      case NormalType::InternalEdge: kind = OLBGFX_INNER_EDGE_VELOCITY; break;
+      case NormalType::InternalCorner: kind = OLBGFX_INNER_CORNER_VELOCITY; break;
       }
+      const CellID c = geometry.getCellId(x, y, z);
+      _slotOfCell[c] = (std::uint8_t)slotFor(kind);
       for (int a = 0; a < 3; ++a) _normals[3 * (std::size_t)c + a] = (std::int8_t)n[a];
     });
     _dirty = true;
@@ -485,11 +531,15 @@ public:
     if (_normals.empty()) {
       _normals.assign(3 * _ncells, 0);
     }
-    geometry.forCore([&](int x, int y, int z) {
+    geometry.forSpatialLocations(2, [&](int x, int y, int z) {
       if (geometry.get(x, y, z) != material) return;
       const auto [type, n] = discreteNormal(geometry, x, y, z);
       if (type != NormalType::Flat) {
         throw std::runtime_error("No valid discrete normal found. This BC is not suited for curved walls.");
+      }
+      if (n[0] == 0 && n[1] == 0 && n[2] == 0) {
+        // boundary/setBoundary3D.h:204-226
+        throw std::runtime_error("Could not set Boundary.");
       }
       const CellID c = geometry.getCellId(x, y, z);
       _slotOfCell[c] = (std::uint8_t)face;

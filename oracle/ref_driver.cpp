@@ -26,6 +26,10 @@
  *                                                stops after that many steps instead
  *   olb_ref poiseuille_local_nonforced N OUT      its non-forced local row (LocalVelocity wall and
  *                                                inflow, LocalPressure outflow)
+ *   olb_ref normals IN OUT                       discrete normals (boundary::set's classifier)
+ *                                                of materials 2 and 3 on the geometry in IN
+ *   olb_ref lv_geometry IN STEPS ULID RE OUT     LocalVelocity walls (2) and lid (3) on the
+ *                                                geometry in IN, BounceBack where no normal
  *   olb_ref bench_tgv N STEPS                    CPU throughput of the TGV step (MLUPS)
  *   olb_ref checkpoint N STEPS OUT NAME           TGV after STEPS steps: populations dump and
  *                                                OpenLB's own checkpoint (SuperLattice::save)
@@ -41,6 +45,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -413,6 +418,112 @@ int run_porous(int nx, int ny, int nz, const std::string& spheres, double radius
 // Taylor-Green vortex (u0 = 0.04, tau = 0.56) after STEPS steps, dumped and
 // written as VTK through a converter with non-unit factors: dx = 1, dt = 0.5,
 // physical density 1.2 (velocity factor 2, pressure factor 4.8)
+// discrete normals on a given geometry: IN holds int32 nx, ny, nz and then
+// the uint8 materials of the core cells (x slowest, z fastest); the block
+// is non-periodic with overlap 3, so the overlap cells stay material 0.
+// OUT gets the padded materials and the normals of materials 2 and 3 as
+// boundary::set computes them (fluid = 1, outside = 0)
+struct GeometryFile {
+  int32_t n[3];
+  std::vector<uint8_t> core;
+};
+
+GeometryFile read_geometry(const std::string& in)
+{
+  GeometryFile g;
+  std::ifstream f(in, std::ios::binary);
+  f.read(reinterpret_cast<char*>(g.n), sizeof(g.n));
+  if (!f || g.n[0] < 1 || g.n[1] < 1 || g.n[2] < 1) {
+    throw std::runtime_error("bad geometry file " + in);
+  }
+  g.core.resize((size_t)g.n[0] * g.n[1] * g.n[2]);
+  f.read(reinterpret_cast<char*>(g.core.data()), g.core.size());
+  if (!f) {
+    throw std::runtime_error("short geometry file " + in);
+  }
+  return g;
+}
+
+// a single non-periodic cuboid of the file's extent (dx = 1, overlap 3)
+// holding the file's core materials
+struct LoadedGeometry {
+  std::unique_ptr<CuboidDecomposition3D<T>> cuboids;
+  std::unique_ptr<HeuristicLoadBalancer<T>> balancer;
+  std::unique_ptr<SuperGeometry<T,3>> G;
+};
+
+LoadedGeometry load_geometry(const GeometryFile& g)
+{
+  LoadedGeometry L;
+  IndicatorCuboid3D<T> mesh(Vector<T,3>(g.n[0] - T(0.5), g.n[1] - T(0.5), g.n[2] - T(0.5)), Vector<T,3>((T)0));
+  L.cuboids = std::make_unique<CuboidDecomposition3D<T>>(mesh, T(1), 1);
+  L.balancer = std::make_unique<HeuristicLoadBalancer<T>>(*L.cuboids);
+  L.G = std::make_unique<SuperGeometry<T,3>>(*L.cuboids, *L.balancer, 3);
+  auto& bg = L.G->getBlockGeometry(0);
+  if (bg.getNx() != g.n[0] || bg.getNy() != g.n[1] || bg.getNz() != g.n[2]) {
+    throw std::runtime_error("cuboid extent does not match the geometry file");
+  }
+  size_t k = 0;
+  for (int x = 0; x < g.n[0]; ++x) {
+    for (int y = 0; y < g.n[1]; ++y) {
+      for (int z = 0; z < g.n[2]; ++z) {
+        bg.set({x, y, z}, g.core[k++]);
+      }
+    }
+  }
+  return L;
+}
+
+// LocalVelocity cavity on the geometry in IN: BGK fluid (1), walls (2) at
+// rest and lid (3) moving at (ULID, 0, 0), both BounceBack where
+// boundary::set<LocalVelocity> finds no discrete normal; nu = ULID (nx-1) /
+// RE; populations, normals and materials after STEPS steps into OUT
+int run_lv_geometry(const std::string& in, int steps, double ulid, double re, const std::string& out)
+{
+  using DESCRIPTOR = descriptors::D3Q19<>;
+  const GeometryFile g = read_geometry(in);
+  LoadedGeometry geo = load_geometry(g);
+  SuperGeometry<T,3>& G = *geo.G;
+  const int res = g.n[0] - 1;
+  const T nu = T(ulid * res / re);
+  SuperLattice<T,DESCRIPTOR> L(lattice_units<DESCRIPTOR>(nu), G);
+  L.setStatisticsOff();
+  dynamics::set<BGKdynamics>(L, G, 1);
+  boundary::set<boundary::BounceBack>(L, G, 2);
+  boundary::set<boundary::BounceBack>(L, G, 3);
+  boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR>>>(L, G, 2);
+  boundary::set<boundary::LocalVelocity<T,DESCRIPTOR,BGKdynamics<T,DESCRIPTOR>>>(L, G, 3);
+  AnalyticalConst3D<T,T> uLid(T(ulid), T(0), T(0));
+  L.defineU(G.getMaterialIndicator(3), uLid);
+  const T omega = T(1) / (T(3) * nu + T(0.5));
+  L.setParameter<descriptors::OMEGA>(omega);
+  L.initialize();
+
+  Dump d(out, block_header(L) + " case=lv_geometry res=" + std::to_string(res) + " steps=" +
+                  std::to_string(steps) + " omega=" + std::to_string((double)omega) + " ulid=" +
+                  std::to_string(ulid) + " nu=" + std::to_string((double)nu));
+  d.array("materials", "i4", materials(G, 3));
+  dump_normals(d, G, 3, {2, 3});
+  d.array("distances", "i8", distances(L));
+  d.array("f0", tname(), populations(L));
+  for (int t = 0; t < steps; ++t) {
+    L.collideAndStream();
+  }
+  d.array("f1", tname(), populations(L));
+  return 0;
+}
+
+int run_normals(const std::string& in, const std::string& out)
+{
+  const GeometryFile g = read_geometry(in);
+  LoadedGeometry geo = load_geometry(g);
+  Dump d(out, "nx=" + std::to_string(g.n[0]) + " ny=" + std::to_string(g.n[1]) + " nz=" +
+                  std::to_string(g.n[2]) + " padding=3 case=normals");
+  d.array("materials", "i4", materials(*geo.G, 3));
+  dump_normals(d, *geo.G, 3, {2, 3});
+  return 0;
+}
+
 int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir)
 {
   using DESCRIPTOR = descriptors::D3Q19<>;
@@ -801,6 +912,12 @@ int main(int argc, char** argv)
     if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && (argc == 4 || argc == 5)) {
       return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local",
                                   argc == 5 ? std::stoi(argv[4]) : -1);
+    }
+    if (mode == "lv_geometry" && argc == 7) {
+      return run_lv_geometry(argv[2], std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6]);
+    }
+    if (mode == "normals" && argc == 4) {
+      return run_normals(argv[2], argv[3]);
     }
     if (mode == "poiseuille_bouzidi" && argc == 4) {
       return run_poiseuille(std::stoi(argv[2]), argv[3], true);

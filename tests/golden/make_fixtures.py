@@ -235,6 +235,10 @@ def main() -> None:
                 vtk_fixtures()
             elif name == "poiseuille_local":
                 poiseuille_local()
+            elif name == "normals":
+                normals_fixture()
+            elif name == "step_cavity":
+                step_cavity_fixture()
             else:
                 full_size(name)
         return
@@ -393,6 +397,110 @@ def poiseuille_bouzidi() -> None:
     }
     np.savez_compressed(os.path.join(HERE, "poiseuille11_forced_bouzidi_f64.npz"), **out)
     print("wrote poiseuille bouzidi", meta)
+
+
+
+def normal_geometries(seed: int = 20261202):
+    """core materials of the classifier geometries: unions of boxes with a
+    wall shell (some walls material 3, some stray wall cells left far from
+    the fluid), random voxel noise, and spheres in a shell (curved walls);
+    together they reach every pattern of the classifier"""
+    rng = np.random.default_rng(seed)
+    out = []
+
+    def shell(fluid, keep=0.0):
+        m = np.where(fluid, 1, 0).astype(np.uint8)
+        near = np.zeros_like(fluid)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    near |= np.roll(np.roll(np.roll(fluid, dx, 0), dy, 1), dz, 2)
+        m[near & ~fluid] = 2
+        far = ~near & ~fluid
+        m[far & (rng.random(m.shape) < keep)] = 2
+        return m
+
+    for _ in range(4):
+        n = rng.integers(12, 17, size=3)
+        fluid = np.zeros(n, bool)
+        for _ in range(rng.integers(2, 5)):
+            lo = [rng.integers(1, s - 3) for s in n]
+            hi = [rng.integers(l + 2, s - 1) for l, s in zip(lo, n)]
+            fluid[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = True
+        m = shell(fluid, keep=0.3)
+        m[(m == 2) & (rng.random(m.shape) < 0.2)] = 3
+        out.append(("boxes", m))
+    for _ in range(2):
+        n = rng.integers(10, 14, size=3)
+        out.append(("noise", rng.choice(np.array([0, 1, 2, 3], np.uint8), size=n, p=[0.1, 0.5, 0.3, 0.1])))
+    for _ in range(2):
+        n = rng.integers(14, 18, size=3)
+        c = (n - 1) / 2 + rng.uniform(-0.5, 0.5, 3)
+        r = rng.uniform(4.0, min(n) / 2 - 1.5)
+        x, y, z = np.meshgrid(*[np.arange(s) for s in n], indexing="ij")
+        out.append(("sphere", shell((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 < r * r)))
+    return out
+
+
+def normals_fixture() -> None:
+    """the reference classifier's types and normals (materials 2 and 3,
+    fluid 1, outside 0) on the geometries of normal_geometries, overlap 3"""
+    arrays = {}
+    kinds = []
+    for k, (kind, m) in enumerate(normal_geometries()):
+        src = os.path.join(RUN, f"normals_geo{k}.bin")
+        with open(src, "wb") as f:
+            f.write(np.asarray(m.shape, dtype=np.int32).tobytes())
+            f.write(np.ascontiguousarray(m, dtype=np.uint8).tobytes())
+        path = os.path.join(RUN, f"normals_{k}.bin")
+        run("olb_ref_d", ["normals", src, path], threads=1)
+        meta, a = read_dump(path)
+        padded = np.pad(m, 3)
+        if not np.array_equal(a["materials"].reshape(padded.shape), padded):
+            raise RuntimeError("the reference's padded materials differ from the geometry written")
+        arrays[f"materials_{k}"] = padded.astype(np.uint8)
+        arrays[f"normal_type_{k}"] = np.asarray(a["normal_type"]).astype(np.int8).reshape(padded.shape)
+        arrays[f"normals_{k}"] = np.asarray(a["normals"]).astype(np.int8).reshape(padded.shape + (3,))
+        kinds.append(kind)
+    arrays["meta"] = np.array(json.dumps({"kinds": kinds, "padding": 3, "boundary": [2, 3], "fluid": 1,
+                                          "outside": 0, "seed": 20261202}))
+    np.savez_compressed(os.path.join(HERE, "discrete_normals_geometries.npz"), **arrays)
+    print("wrote discrete_normals_geometries", kinds)
+
+
+
+def step_cavity_fixture(res: int = 10, steps: int = 100) -> None:
+    """the reference's LocalVelocity cavity with a solid corner cube
+    (cases.step_cavity: every LocalVelocity shape, BounceBack where no
+    normal) in FP64: t = 0 and t = steps populations of the core cells,
+    materials and the reference's own normals"""
+    c = cases.step_cavity(res)
+    p = c.padding
+    core_m = np.ascontiguousarray(c.materials[p:-p, p:-p, p:-p])
+    src = os.path.join(RUN, "step_cavity_geo.bin")
+    with open(src, "wb") as f:
+        f.write(np.asarray(core_m.shape, dtype=np.int32).tobytes())
+        f.write(core_m.astype(np.uint8).tobytes())
+    path = os.path.join(RUN, "step_cavity.bin")
+    run("olb_ref_d", ["lv_geometry", src, str(steps), "0.1", "100", path])
+    meta, a = read_dump(path)
+    if not np.array_equal(a["materials"], c.materials.reshape(-1)):
+        raise RuntimeError("the reference's padded materials differ from the case")
+    q = 19
+    n = a["f0"].size // q
+    core = np.flatnonzero(c.core_mask().reshape(-1))
+    out = {
+        "meta": np.array(json.dumps({**meta, "step": c.info["step"], "u_lid": 0.1, "re": 100.0})),
+        "materials": c.materials.reshape(-1).astype(np.uint8),
+        "distances": a["distances"],
+        "cells": core.astype(np.int64),
+        "f0": a["f0"].reshape(q, n)[:, core],
+        "f1": a["f1"].reshape(q, n)[:, core],
+        "normal_type": np.asarray(a["normal_type"]).astype(np.int8),
+        "normals": np.asarray(a["normals"]).astype(np.int8),
+    }
+    np.savez_compressed(os.path.join(HERE, f"step_cavity_res{res}_f64.npz"), **out)
+    print("wrote step_cavity", meta)
 
 
 if __name__ == "__main__":

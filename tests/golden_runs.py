@@ -327,3 +327,36 @@ def local_velocity_boundary_errors(f: np.ndarray, d) -> tuple:
         else:
             edge_err = max(edge_err, e)
     return flat_err, edge_err
+
+
+def step_cavity_oracle(storage: str):
+    """the step cavity of the fixture built as a Case (its discrete normals
+    from the classifier, not from the fixture) and advanced by the oracle,
+    with the mask of populations that do not depend on what the padding and
+    the outside (material 0) cells hold: two more oracle runs refill those
+    cells with different random values before every step, and a population
+    is defined where the two agree.  Returns (case, fixture, core cells,
+    oracle populations of the core cells, defined mask)."""
+    from oracle.oracle import OracleLattice
+    from openlb_amd import cases
+
+    meta, d = load("step_cavity_res10_f64")
+    dtype = np.float64 if storage == "f64" else np.float32
+    c = cases.step_cavity(meta["res"], u_lid=meta["u_lid"], re=meta["re"], step=meta["step"], dtype=dtype)
+    cells = d["cells"]
+    garbage = ~c.core_mask().reshape(-1) | (c.materials.reshape(-1) == 0)
+    runs = []
+    for seed in (None, 1, 2):
+        o = OracleLattice(c.extent, c.padding, c.periodic, storage)
+        c.apply(o)
+        if seed is None:
+            o.collide_and_stream(meta["steps"])
+        else:
+            rng = np.random.default_rng(seed)
+            for _ in range(meta["steps"]):
+                f = o.get_populations()
+                f[:, garbage] = rng.uniform(-0.01, 0.01, size=(19, int(garbage.sum())))
+                o.set_populations(f)
+                o.collide_and_stream(1)
+        runs.append(np.asarray(o.get_populations()[:, cells], dtype=np.float64))
+    return c, d, cells, runs[0], runs[1] == runs[2]

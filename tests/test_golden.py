@@ -1,6 +1,9 @@
 """The oracle and the case builders against the reference's own CPU path
 (fixtures in tests/golden/, written by OpenLB through oracle/_ref) and the
 reference's known-answer test (Poiseuille, forced BGK + BounceBack)."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -8,7 +11,10 @@ from openlb_amd import cases
 from oracle.oracle import OracleLattice
 from tests.golden_runs import (build, compared_cells, core_fluid, load, local_velocity_boundary_errors,
                                value_tracer_run, velocity_error_l2)
+from tests.helpers import T as T_W
 from tests.helpers import rel_err_unshifted
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 FIXTURES = ["cavity3d_res12", "cavity3d_lv_res12", "tgv12", "tgv12_smagorinsky", "porous16x12x12"]
 TOL = {"f32": 1e-5, "f64": 1e-12}
@@ -212,3 +218,43 @@ def test_oracle_poiseuille_forced_local_known_answer():
     print("velocity", err, "checker", meta["velocityErrorL2"])
     assert abs(err - 0.020342) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 3e-6
+
+
+def test_classifier_matches_reference_on_random_geometries():
+    """boundary::set's discrete normals (type and normal of every cell of
+    materials 2 and 3 at least two cells inside the block) on unions of
+    boxes, voxel noise and spheres in a shell, against the reference's own
+    classifier run on the same materials; every pattern of the classifier
+    decides at least one cell"""
+    from openlb_amd.normals import RULES, discrete_normals
+
+    d = np.load(os.path.join(GOLDEN, "discrete_normals_geometries.npz"))
+    meta = json.loads(str(d["meta"]))
+    hits = np.zeros(len(RULES), dtype=int)
+    for k, kind in enumerate(meta["kinds"]):
+        m = d[f"materials_{k}"]
+        types, normals, rule = discrete_normals(m, meta["boundary"], with_rule=True)
+        np.testing.assert_array_equal(types, d[f"normal_type_{k}"], err_msg=f"geometry {k} ({kind})")
+        np.testing.assert_array_equal(normals, d[f"normals_{k}"], err_msg=f"geometry {k} ({kind})")
+        hits += np.bincount(rule[rule >= 0], minlength=len(RULES))
+    assert hits.min() > 0
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_oracle_step_cavity_matches_reference(storage, tol):
+    """the LocalVelocity cavity with a solid corner cube (flat faces,
+    external and internal edges and corners, BounceBack where no normal),
+    set up from the classifier's normals, against the reference after 100
+    steps on every defined population of every core cell; FP64 within
+    1e-12, FP32 within 2e-5 relative (the reference ran in FP64)"""
+    from tests.golden_runs import step_cavity_oracle
+
+    c, d, cells, f, defined = step_cavity_oracle(storage)
+    types, _ = c.cell_normals()
+    assert set(np.unique(types[types >= 0])) == {0, 1, 2, 3, 4}
+    mats = d["materials"][cells]
+    assert defined[:, mats == 1].all()
+    ref = d["f1"]
+    rel = np.where(defined, np.abs(f - ref) / np.abs(ref + T_W[:, None]), 0.0)
+    print(storage, "step cavity oracle vs reference", float(rel.max()))
+    assert rel.max() <= tol

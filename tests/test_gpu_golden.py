@@ -2,6 +2,8 @@
 by OpenLB through oracle/_ref) and against the reference's known-answer test.
 Tolerances: FP32 populations within 1e-5 relative (unshifted), FP64 within
 1e-12; the known-answer test to its own 1e-6 and the exact iteration count."""
+import json
+
 import numpy as np
 import pytest
 
@@ -320,3 +322,31 @@ def test_gpu_poiseuille_nonforced_local_storage(storage):
     err = rel_l2(ug, ur)
     print(storage, "nonforced local velocity rel L2 vs reference FP64", err)
     assert err <= (1e-5 if storage == "f32" else 2e-3)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_gpu_step_cavity_matches_oracle_and_reference(storage):
+    """the LocalVelocity cavity with a solid corner cube (every
+    LocalVelocity shape: flat faces, external and internal edges and
+    corners, BounceBack where no normal), set up from the classifier's
+    normals, after 100 steps: every defined population of every core cell
+    against the oracle (FP32 within 1e-5, FP64 within 1e-12 relative) and,
+    in FP64, against the reference within 1e-12"""
+    from tests.golden_runs import step_cavity_oracle
+    from tests.helpers import T as W
+
+    c, d, cells, ref_orc, defined = step_cavity_oracle(storage)
+    print(f"[golden] step cavity {storage}", flush=True)
+    lat = Lattice(c.extent, c.padding, c.periodic, storage)
+    c.apply(lat)
+    lat.collide_and_stream(int(json.loads(str(d["meta"]))["steps"]))
+    got = np.asarray(lat.get_populations()[:, cells], dtype=np.float64)
+    lat.close()
+    rel = np.where(defined, np.abs(got - ref_orc) / np.abs(ref_orc + W[:, None]), 0.0)
+    print(storage, "step cavity gpu vs oracle", float(rel.max()))
+    assert rel.max() <= TOL[storage]
+    if storage == "f64":
+        ref = d["f1"]
+        rel = np.where(defined, np.abs(got - ref) / np.abs(ref + W[:, None]), 0.0)
+        print("step cavity gpu vs reference", float(rel.max()))
+        assert rel.max() <= 1e-12

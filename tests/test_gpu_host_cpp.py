@@ -136,3 +136,31 @@ def test_cpp_channel_matches_oracle(tmp_path):
     err = rel_err_unshifted(f[:, sel], orc.get_populations()[:, sel])
     print("channel fluid vs oracle", err)
     assert err <= 1e-5
+
+
+def test_cpp_step_cavity_matches_reference(tmp_path):
+    """the C++ mirror's cavity with a solid corner cube (--STEP: its
+    classifier gives the walls every LocalVelocity shape, BounceBack where no
+    normal) after 10 + 90 steps against the reference's FP64 run of the same
+    case on every defined population of every core cell (FP32 storage,
+    within 2e-5 relative)"""
+    import json
+
+    from tests.golden_runs import step_cavity_oracle
+    from tests.helpers import T as W
+
+    c, d, cells, _, defined = step_cavity_oracle("f32")
+    meta = json.loads(str(d["meta"]))
+    dump = tmp_path / "step.bin"
+    print("[cpp] cavity3d step", flush=True)
+    out = subprocess.run([BINARY, "--RESOLUTION", str(meta["res"]), "--TIME_STEPS", str(meta["steps"] - 10),
+                          "--STEP", str(meta["step"]), "--DUMP", str(dump)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    raw = np.fromfile(dump, dtype=np.uint8)
+    np.testing.assert_array_equal(raw[:c.ncells], d["materials"])
+    f = raw[c.ncells:].view(np.float32).reshape(19, c.ncells)[:, cells].astype(np.float64)
+    ref = d["f1"]
+    rel = np.where(defined, np.abs(f - ref) / np.abs(ref + W[:, None]), 0.0)
+    print("cpp step cavity vs reference", float(rel.max()))
+    assert rel.max() <= 2e-5
