@@ -84,6 +84,18 @@ typedef int32_t olbgfx_status;
 #define OLBGFX_INNER_EDGE_VELOCITY   10
 #define OLBGFX_INNER_CORNER_VELOCITY 11
 #define OLBGFX_LOCAL_PRESSURE        12
+/* InterpolatedVelocity and InterpolatedPressure faces
+ * (boundary/interpolatedVelocity3D.h:45-100, interpolatedPressure3D.h:45-85),
+ * flat normal only: BGK through BasicDirichletVelocityBoundaryTuple (density
+ * from the known populations, the wall VELOCITY) or
+ * BasicDirichletPressureBoundaryTuple (fixed density, normal velocity from
+ * the known populations) (momenta/aliases.h:173-198), and after streaming
+ * PlaneFdBoundaryProcessor3D (boundary/boundaryPostProcessors3D.hh:39-92):
+ * every population set to feq + fneq of the finite-difference strain.  The
+ * edges and corners of InterpolatedVelocity are the LocalVelocity kinds
+ * (FIXED_VELOCITY, INNER_EDGE_VELOCITY, INNER_CORNER_VELOCITY). */
+#define OLBGFX_INTERPOLATED_VELOCITY 13
+#define OLBGFX_INTERPOLATED_PRESSURE 14
 
 /* per-cell fields */
 #define OLBGFX_FIELD_VELOCITY 0  /* descriptors::VELOCITY, 3 values per cell */

@@ -32,6 +32,7 @@ VALUE_F32, VALUE_F64 = 0, 1
 NO_DYNAMICS, BGK, TRT, FORCED_BGK, FORCED_TRT, BOUNCE_BACK, BOUNCE_BACK_VELOCITY = range(7)
 LOCAL_VELOCITY, FIXED_VELOCITY, SMAGORINSKY_BGK = 7, 8, 9
 INNER_EDGE_VELOCITY, INNER_CORNER_VELOCITY, LOCAL_PRESSURE = 10, 11, 12
+INTERPOLATED_VELOCITY, INTERPOLATED_PRESSURE = 13, 14
 FIELD_VELOCITY, FIELD_RHO, FIELD_FORCE = 0, 1, 2
 MAX_DYNAMICS = 16
 Q = 19

@@ -18,7 +18,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import (BGK, BOUNCE_BACK, BOUNCE_BACK_VELOCITY, FIELD_VELOCITY, FIXED_VELOCITY, FORCED_TRT,
-               INNER_CORNER_VELOCITY, INNER_EDGE_VELOCITY, LOCAL_PRESSURE, LOCAL_VELOCITY, NO_DYNAMICS, Dynamics)
+               INNER_CORNER_VELOCITY, INNER_EDGE_VELOCITY, INTERPOLATED_PRESSURE, INTERPOLATED_VELOCITY,
+               LOCAL_PRESSURE, LOCAL_VELOCITY, NO_DYNAMICS, Dynamics)
 from .normals import EXTERNAL_CORNER, EXTERNAL_EDGE, FLAT, INTERNAL_CORNER, INTERNAL_EDGE, discrete_normals
 
 __all__ = [
@@ -73,17 +74,10 @@ class Case:
         normals = self.cell_normals()
         for mat in sorted(self.dynamics):
             d = self.dynamics[mat]
-            if isinstance(d, (LocalVelocity, LocalPressure)):
+            if isinstance(d, _BOUNDARIES):
                 on = self.materials == mat
                 types = np.where(normals[1][on].any(axis=-1) | (normals[0][on] != FLAT), normals[0][on], -1)
-                if isinstance(d, LocalVelocity):
-                    shapes = ((FLAT, LOCAL_VELOCITY), (EXTERNAL_EDGE, FIXED_VELOCITY),
-                              (EXTERNAL_CORNER, FIXED_VELOCITY), (INTERNAL_EDGE, INNER_EDGE_VELOCITY),
-                              (INTERNAL_CORNER, INNER_CORNER_VELOCITY))
-                    extra = {"wall_velocity": d.wall_velocity}
-                else:
-                    shapes = ((FLAT, LOCAL_PRESSURE),)
-                    extra = {"wall_rho": d.rho}
+                shapes, extra = d.shapes(), d.extra()
                 by_kind = {}
                 for t, kind in shapes:
                     if not np.any(types == t):
@@ -103,8 +97,10 @@ class Case:
         return slots, table
 
     def local_velocity_materials(self) -> tuple:
-        """the materials with local boundaries (LocalVelocity or LocalPressure)"""
-        return tuple(m for m, d in sorted(self.dynamics.items()) if isinstance(d, (LocalVelocity, LocalPressure)))
+        """the materials with boundaries set through discrete normals
+        (LocalVelocity, LocalPressure, InterpolatedVelocity,
+        InterpolatedPressure)"""
+        return tuple(m for m, d in sorted(self.dynamics.items()) if isinstance(d, _BOUNDARIES))
 
     def cell_normals(self):
         """(type, normal) per cell for the LocalVelocity materials, or None"""
@@ -184,6 +180,13 @@ class LocalVelocity:
     # discrete normal; boundary::set leaves what defineDynamics gave them)
     base: object = None
 
+    def shapes(self):
+        return ((FLAT, LOCAL_VELOCITY), (EXTERNAL_EDGE, FIXED_VELOCITY), (EXTERNAL_CORNER, FIXED_VELOCITY),
+                (INTERNAL_EDGE, INNER_EDGE_VELOCITY), (INTERNAL_CORNER, INNER_CORNER_VELOCITY))
+
+    def extra(self):
+        return {"wall_velocity": self.wall_velocity}
+
 
 @dataclass
 class LocalPressure:
@@ -195,6 +198,54 @@ class LocalPressure:
     omega: float
     rho: float = 1.0
     base: object = None
+
+    def shapes(self):
+        return ((FLAT, LOCAL_PRESSURE),)
+
+    def extra(self):
+        return {"wall_rho": self.rho}
+
+
+@dataclass
+class InterpolatedVelocity:
+    """boundary::set<boundary::InterpolatedVelocity<T, D, BGKdynamics>>
+    (boundary/interpolatedVelocity3D.h:45-100): flat faces collide as BGK
+    with BasicDirichletVelocityBoundaryTuple and run PlaneFdBoundaryProcessor3D
+    after streaming; edges and corners as LocalVelocity's.  Cells without a
+    discrete normal keep `base`."""
+
+    omega: float
+    wall_velocity: tuple = (0.0, 0.0, 0.0)
+    base: object = None
+
+    def shapes(self):
+        return ((FLAT, INTERPOLATED_VELOCITY), (EXTERNAL_EDGE, FIXED_VELOCITY), (EXTERNAL_CORNER, FIXED_VELOCITY),
+                (INTERNAL_EDGE, INNER_EDGE_VELOCITY), (INTERNAL_CORNER, INNER_CORNER_VELOCITY))
+
+    def extra(self):
+        return {"wall_velocity": self.wall_velocity}
+
+
+@dataclass
+class InterpolatedPressure:
+    """boundary::set<boundary::InterpolatedPressure<T, D, BGKdynamics>>
+    (boundary/interpolatedPressure3D.h:45-85): flat faces collide as BGK
+    with BasicDirichletPressureBoundaryTuple (density `rho` or the RHO
+    field) and run PlaneFdBoundaryProcessor3D; every other cell keeps
+    `base`."""
+
+    omega: float
+    rho: float = 1.0
+    base: object = None
+
+    def shapes(self):
+        return ((FLAT, INTERPOLATED_PRESSURE),)
+
+    def extra(self):
+        return {"wall_rho": self.rho}
+
+
+_BOUNDARIES = (LocalVelocity, LocalPressure, InterpolatedVelocity, InterpolatedPressure)
 
 
 def _padded(extent, padding):
@@ -482,12 +533,15 @@ def cavity3d(res: int, u_lid: float = 0.1, re: float = 100.0, padding: int = 3,
 
 
 def channel3d(nx: int, ny: int, nz: int, u_in: float = 0.05, tau: float = 0.8, padding: int = 3,
-              dtype=np.float64) -> Case:
+              dtype=np.float64, boundary: str = "local") -> Case:
     """A straight channel along x on an nx x ny x nz core: BounceBack walls
     (material 2) on the y and z faces, the x = 0 face without its rim a
     LocalVelocity inflow at (u_in, 0, 0) (material 3), the x = nx-1 face
     without its rim a LocalPressure outflow at rho = 1 (material 4), BGK fluid
-    (material 1) starting at rest."""
+    (material 1) starting at rest.  boundary="interpolated" makes the inflow
+    InterpolatedVelocity and the outflow InterpolatedPressure."""
+    if boundary not in ("local", "interpolated"):
+        raise ValueError("boundary is local or interpolated")
     extent = (nx, ny, nz)
     shape = _padded(extent, padding)
     p = padding
@@ -505,8 +559,9 @@ def channel3d(nx: int, ny: int, nz: int, u_in: float = 0.05, tau: float = 0.8, p
         periodic=(False, False, False),
         materials=mats,
         dynamics={1: Dynamics(BGK, omega=omega), 2: Dynamics(BOUNCE_BACK),
-                  3: LocalVelocity(omega=omega, wall_velocity=(u_in, 0.0, 0.0)),
-                  4: LocalPressure(omega=omega, rho=1.0)},
+                  3: (InterpolatedVelocity if boundary == "interpolated" else LocalVelocity)(
+                      omega=omega, wall_velocity=(u_in, 0.0, 0.0)),
+                  4: (InterpolatedPressure if boundary == "interpolated" else LocalPressure)(omega=omega, rho=1.0)},
         rho=np.ones(shape, dtype=dtype),
         u=np.zeros(shape + (3,), dtype=dtype),
         init_materials=(1, 2, 3, 4),

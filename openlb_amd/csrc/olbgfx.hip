@@ -52,7 +52,13 @@ constexpr int kMaskedBoundary = -2;  // step-kernel MODE: masked, LocalVelocity 
 __host__ __device__ constexpr bool is_velocity_boundary(int kind)
 {
   return kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_FIXED_VELOCITY || kind == OLBGFX_INNER_EDGE_VELOCITY ||
-         kind == OLBGFX_INNER_CORNER_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE;
+         kind == OLBGFX_INNER_CORNER_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE ||
+         kind == OLBGFX_INTERPOLATED_VELOCITY || kind == OLBGFX_INTERPOLATED_PRESSURE;
+}
+// the flat boundary kinds whose cells run PlaneFdBoundaryProcessor3D
+__host__ __device__ constexpr bool is_plane_fd(int kind)
+{
+  return kind == OLBGFX_INTERPOLATED_VELOCITY || kind == OLBGFX_INTERPOLATED_PRESSURE;
 }
 // LDS of one gfx950 compute unit
 constexpr uint32_t kLdsPerCu = 160 * 1024;
@@ -779,6 +785,28 @@ __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntr
     st_rho = rho;
     st_usqr = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
     return true;
+  } else if (e.kind == OLBGFX_INTERPOLATED_VELOCITY || e.kind == OLBGFX_INTERPOLATED_PRESSURE) {
+    // BGK (collision.h:57-80) through BasicDirichletVelocityBoundaryTuple
+    // (VelocityBoundaryDensity, FixedVelocityMomentumGeneric) or
+    // BasicDirichletPressureBoundaryTuple (FixedDensity,
+    // FixedPressureMomentum), momenta/aliases.h:173-198
+    int dir, orient;
+    flat_normal(a.normal[n], dir, orient);
+    A uc[3] = {u[0], u[1], u[2]};
+    if (e.kind == OLBGFX_INTERPOLATED_VELOCITY) {
+      rho = velocity_bm_rho(f, dir, orient, u[dir]);
+    } else {
+      rho = a.cell_rho ? a.cell_rho[n] : e.wall_rho;
+      pressure_u(f, dir, orient, rho, u, uc);
+    }
+    const A usqr = uc[0] * uc[0] + uc[1] * uc[1] + uc[2] * uc[2];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      f[i] = f[i] + e.omega * (feq_i(i, rho, uc, usqr) - f[i]);
+    }
+    st_rho = rho;
+    st_usqr = usqr;
+    return true;
   } else if (e.kind == OLBGFX_INNER_EDGE_VELOCITY || e.kind == OLBGFX_INNER_CORNER_VELOCITY) {
     const uint32_t code = a.normal[n];
     const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
@@ -966,8 +994,10 @@ __device__ typename Storage<S>::A momenta_rho(const StepArgs<typename Storage<S>
   case OLBGFX_BOUNCE_BACK:
   case OLBGFX_BOUNCE_BACK_VELOCITY:
   case OLBGFX_LOCAL_PRESSURE:
+  case OLBGFX_INTERPOLATED_PRESSURE:
     return a.cell_rho ? a.cell_rho[m] : e.wall_rho;  // FixedDensity
-  case OLBGFX_LOCAL_VELOCITY: {
+  case OLBGFX_LOCAL_VELOCITY:
+  case OLBGFX_INTERPOLATED_VELOCITY: {
     const uint32_t code = a.normal[m];
     for (int ax = 0; ax < 3; ++ax) {
       const int o = normal_component(code, ax);
@@ -1009,9 +1039,11 @@ __device__ void momenta_u(const StepArgs<typename Storage<S>::A>& a, uint32_t m,
   case OLBGFX_INNER_EDGE_VELOCITY:
   case OLBGFX_INNER_CORNER_VELOCITY:
   case OLBGFX_BOUNCE_BACK_VELOCITY:
+  case OLBGFX_INTERPOLATED_VELOCITY:
     for (int ax = 0; ax < 3; ++ax) u[ax] = wall_component(a, e, m, ax);
     return;
-  case OLBGFX_LOCAL_PRESSURE: {
+  case OLBGFX_LOCAL_PRESSURE:
+  case OLBGFX_INTERPOLATED_PRESSURE: {
     A f[Q];
     load_cell<S>(a, m, f);
     int dir, orient;
@@ -1107,6 +1139,68 @@ __global__ void __launch_bounds__(256) k_velocity_post(const StepArgs<typename S
   for (int i = 0; i < Q; ++i) {
     *ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)) =
         Storage<S>::pack(feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi));
+  }
+}
+
+// PlaneFdBoundaryProcessor3D<direction, orientation>
+// (boundary/boundaryPostProcessors3D.hh:39-92) on the flat cells of the
+// interpolated boundaries: density and velocity through the cell's momenta,
+// the velocity gradient one-sided along the normal and central along the
+// wall (utilities/finiteDifference3D.h:75-230), then every population set
+// to the regularised value feq(rho, u) + fneq(Pi(grad u)).  The new
+// populations go to `out` (19 per listed cell) and k_plane_fd_store writes
+// them back, so no processor reads a value another one has rewritten.
+template <typename S>
+__global__ void __launch_bounds__(256) k_plane_fd(const StepArgs<typename Storage<S>::A> a, const uint32_t* cells,
+                                                  uint32_t ncells, typename Storage<S>::A* out)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) {
+    return;
+  }
+  const uint32_t n = cells[t];
+  int dir, orient;
+  flat_normal(a.normal[n], dir, orient);
+  const int64_t step[3] = {(int64_t)a.py * a.pz, (int64_t)a.pz, 1};
+  const DynEntry<A>& e = a.table[a.mask[n]];
+  const A rho = momenta_rho<S>(a, n);
+  A u[3];
+  momenta_u<S>(a, n, u);
+  A du[3][3];
+  for (int d = 0; d < 3; ++d) {
+    if (d == dir) {
+      one_sided<S>(a, n, step[d], orient, du[d]);
+    } else {
+      A up[3], um[3];
+      momenta_u<S>(a, (uint32_t)((int64_t)n + step[d]), up);
+      momenta_u<S>(a, (uint32_t)((int64_t)n - step[d]), um);
+      for (int b = 0; b < 3; ++b) du[d][b] = (up[b] - um[b]) / A(2);
+    }
+  }
+  const A s_to_pi = -rho / A(3) / e.omega;
+  const A pi[6] = {A(2) * du[0][0] * s_to_pi, (du[0][1] + du[1][0]) * s_to_pi, (du[0][2] + du[2][0]) * s_to_pi,
+                   A(2) * du[1][1] * s_to_pi, (du[1][2] + du[2][1]) * s_to_pi, A(2) * du[2][2] * s_to_pi};
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    out[(size_t)i * ncells + t] = feq_i(i, rho, u, usqr) + pi_to_fneq(i, pi);
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_plane_fd_store(const StepArgs<typename Storage<S>::A> a,
+                                                        const uint32_t* cells, uint32_t ncells,
+                                                        const typename Storage<S>::A* out)
+{
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells) {
+    return;
+  }
+  const uint32_t n = cells[t];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    *ring_at<S>(a.pop, a.stride, i, ring_slot(n, a.shift[i], a.count)) = Storage<S>::pack(out[(size_t)i * ncells + t]);
   }
 }
 
@@ -1549,6 +1643,9 @@ struct olbgfx_lattice {
   uint8_t* normal_dev = nullptr;     // code 9(nx+1)+3(ny+1)+(nz+1) per cell
   uint32_t* post_dev = nullptr;
   uint32_t post_edges = 0, post_corners = 0;
+  uint32_t* fd_dev = nullptr;        // flat interpolated cells (PlaneFdBoundaryProcessor3D)
+  void* fd_out = nullptr;            // their new populations, 19 per cell in the arithmetic type
+  uint32_t fd_n = 0;
   uint32_t* refresh_dev = nullptr;   // (periodic padding cell, image) pairs the edge/corner processors read
   uint32_t refresh_n = 0;
   bool boundary_kinds = false;       // some core cell has a LocalVelocity kind
@@ -1667,7 +1764,10 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L);
 // PostStream processors on the populations at the lattice's current offsets:
 // the overlap exchange (exchange = true, as SuperLattice::collideAndStream
 // does before them), Bouzidi (priority -1), LocalVelocity edges (priority 0),
-// then corners (priority 1)
+// then corners (priority 1), then the flat faces of interpolated boundaries
+// (priority 0; their order against the edges changes results only by
+// rounding: an edge reads a face's density, which the regularised
+// populations the face gets reproduce)
 template <typename S>
 olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a, hipStream_t stream,
                           bool exchange)
@@ -1689,6 +1789,12 @@ olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a
   if (L->post_corners) {
     k_velocity_post<S><<<(L->post_corners + 255) / 256, 256, 0, stream>>>(a, L->post_dev + L->post_edges,
                                                                          L->post_corners);
+    HIP_TRY(hipGetLastError());
+  }
+  if (L->fd_n) {
+    k_plane_fd<S><<<(L->fd_n + 255) / 256, 256, 0, stream>>>(a, L->fd_dev, L->fd_n, (A*)L->fd_out);
+    HIP_TRY(hipGetLastError());
+    k_plane_fd_store<S><<<(L->fd_n + 255) / 256, 256, 0, stream>>>(a, L->fd_dev, L->fd_n, (const A*)L->fd_out);
     HIP_TRY(hipGetLastError());
   }
   return OLBGFX_OK;
@@ -1882,7 +1988,7 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
   for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
     any = any || is_velocity_boundary(L->table[k].kind);
   }
-  std::vector<uint32_t> edges, corners;
+  std::vector<uint32_t> edges, corners, fd;
   bool used = false;
   const olbgfx_lattice_desc& D = L->desc;
   const int P[3] = {D.nx + 2 * D.padding, D.ny + 2 * D.padding, D.nz + 2 * D.padding};
@@ -1904,8 +2010,19 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
             nz += N[a] != 0;
           }
         }
-        if ((kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE) && nz != 1) {
-          return fail(OLBGFX_ERR_INVALID, "a LOCAL_VELOCITY or LOCAL_PRESSURE cell needs a flat discrete normal");
+        if ((kind == OLBGFX_LOCAL_VELOCITY || kind == OLBGFX_LOCAL_PRESSURE || is_plane_fd(kind)) && nz != 1) {
+          return fail(OLBGFX_ERR_INVALID,
+                      "a LOCAL_VELOCITY, LOCAL_PRESSURE or INTERPOLATED_* cell needs a flat discrete normal");
+        }
+        if (is_plane_fd(kind)) {
+          // reach: two cells inwards along the normal, one along the wall
+          for (int a = 0; a < 3; ++a) {
+            const int lo = N[a] > 0 ? 2 : 1, hi = N[a] < 0 ? 2 : 1;
+            if (c[a] - lo < 0 || c[a] + hi >= P[a]) {
+              return fail(OLBGFX_ERR_INVALID, "an interpolated boundary processor reaches outside the block");
+            }
+          }
+          fd.push_back(n);
         }
         if (kind == OLBGFX_INNER_EDGE_VELOCITY && nz != 2) {
           return fail(OLBGFX_ERR_INVALID, "an INNER_EDGE_VELOCITY cell needs a normal with two non-zero entries");
@@ -1989,6 +2106,7 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
     };
     for (uint32_t n : edges) visit(n);
     for (uint32_t n : corners) visit(n);
+    for (uint32_t n : fd) visit(n);
   }
   if (L->refresh_dev) {
     HIP_TRY(hipFree(L->refresh_dev));
@@ -2009,6 +2127,21 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
     edges.insert(edges.end(), corners.begin(), corners.end());
     HIP_TRY(hipMalloc(&L->post_dev, sizeof(uint32_t) * edges.size()));
     HIP_TRY(hipMemcpy(L->post_dev, edges.data(), sizeof(uint32_t) * edges.size(), hipMemcpyHostToDevice));
+  }
+  if (L->fd_dev) {
+    HIP_TRY(hipFree(L->fd_dev));
+    L->fd_dev = nullptr;
+  }
+  if (L->fd_out) {
+    HIP_TRY(hipFree(L->fd_out));
+    L->fd_out = nullptr;
+  }
+  L->fd_n = (uint32_t)fd.size();
+  if (!fd.empty()) {
+    const size_t value = L->desc.storage == OLBGFX_STORE_F64 ? sizeof(double) : sizeof(float);
+    HIP_TRY(hipMalloc(&L->fd_dev, sizeof(uint32_t) * fd.size()));
+    HIP_TRY(hipMemcpy(L->fd_dev, fd.data(), sizeof(uint32_t) * fd.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&L->fd_out, value * Q * fd.size()));
   }
   L->boundary_kinds = used;
   L->boundary_version = L->version;
@@ -2118,6 +2251,8 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->cell_force);
   hipFree(L->normal_dev);
   hipFree(L->post_dev);
+  hipFree(L->fd_dev);
+  hipFree(L->fd_out);
   hipFree(L->refresh_dev);
   hipFree(L->bz_cells);
   hipFree(L->bz_q);
@@ -2166,7 +2301,7 @@ olbgfx_status olbgfx_set_dynamics(olbgfx_lattice* L, uint32_t id, const olbgfx_d
   if (id >= OLBGFX_MAX_DYNAMICS) {
     return fail(OLBGFX_ERR_INVALID, "dynamics id out of range");
   }
-  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_LOCAL_PRESSURE) {
+  if (dyn->kind < OLBGFX_NO_DYNAMICS || dyn->kind > OLBGFX_INTERPOLATED_PRESSURE) {
     return fail(OLBGFX_ERR_UNSUPPORTED, "dynamics kind not provided by the gfx950 platform");
   }
   L->table[id] = *dyn;

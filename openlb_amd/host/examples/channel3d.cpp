@@ -6,7 +6,11 @@
  * pair of the reference's non-forced Poiseuille test), BGK fluid at rest.
  *
  *   channel3d [--NX n] [--NY n] [--NZ n] [--TIME_STEPS k] [--U_IN u] [--TAU t]
- *             [--DUMP file]
+ *             [--DUMP file] [--BOUNDARY local|interpolated]
+ *
+ * --BOUNDARY interpolated uses boundary::InterpolatedVelocity and
+ * boundary::InterpolatedPressure instead (the pair of the test's
+ * interpolated rows).
  *
  * --DUMP writes the materials and the populations of the padded block after
  * the run (logical order, float32) for the tests.
@@ -27,6 +31,7 @@ int main(int argc, char** argv)
     int nx = 48, ny = 16, nz = 16, steps = 100;
     T uIn = T(0.05), tau = T(0.8);
     std::string dump;
+    bool interpolated = false;
     for (int i = 1; i + 1 < argc; i += 2) {
       const std::string k = argv[i], v = argv[i + 1];
       if (k == "--NX") nx = std::stoi(v);
@@ -36,6 +41,10 @@ int main(int argc, char** argv)
       else if (k == "--U_IN") uIn = std::stof(v);
       else if (k == "--TAU") tau = std::stof(v);
       else if (k == "--DUMP") dump = v;
+      else if (k == "--BOUNDARY") {
+        if (v != "local" && v != "interpolated") throw std::invalid_argument("--BOUNDARY is local or interpolated");
+        interpolated = v == "interpolated";
+      }
       else throw std::invalid_argument("unknown option " + k);
     }
     SuperGeometry<T> geometry({nx, ny, nz}, 3);
@@ -47,8 +56,13 @@ int main(int argc, char** argv)
     SuperLattice<T, DESCRIPTOR> lattice(geometry);
     dynamics::set<BGKdynamics>(lattice, geometry, 1);
     boundary::set<boundary::BounceBack>(lattice, geometry, 2);
-    boundary::set<boundary::LocalVelocity>(lattice, geometry, 3);
-    boundary::set<boundary::LocalPressure>(lattice, geometry, 4);
+    if (interpolated) {
+      boundary::set<boundary::InterpolatedVelocity>(lattice, geometry, 3);
+      boundary::set<boundary::InterpolatedPressure>(lattice, geometry, 4);
+    } else {
+      boundary::set<boundary::LocalVelocity>(lattice, geometry, 3);
+      boundary::set<boundary::LocalPressure>(lattice, geometry, 4);
+    }
     lattice.defineU(geometry, 3, {uIn, T(0), T(0)});
     lattice.defineRho(geometry, 4, T(1));
     lattice.setParameter<descriptors::OMEGA>(T(1) / tau);

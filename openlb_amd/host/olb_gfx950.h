@@ -483,8 +483,11 @@ public:
   /// regularised velocity cells, external edges and corners fixed-velocity
   /// BGK cells with their post-processors, internal edges and corners the
   /// inner-edge and inner-corner velocity cells.  A cell without a discrete
-  /// normal keeps the dynamics it had, as in the reference.
-  void setLocalVelocity(SuperGeometry<T>& geometry, int material)
+  /// normal keeps the dynamics it had, as in the reference.  With
+  /// flatKind = OLBGFX_INTERPOLATED_VELOCITY the flat faces are those of
+  /// boundary::InterpolatedVelocity instead (BGK faces with the plane
+  /// finite-difference processor, boundary/interpolatedVelocity3D.h:45-100).
+  void setLocalVelocity(SuperGeometry<T>& geometry, int material, int flatKind = OLBGFX_LOCAL_VELOCITY)
   {
     if (_normals.empty()) {
       _normals.assign(3 * _ncells, 0);
@@ -493,9 +496,9 @@ public:
       if (geometry.get(x, y, z) != material) return;
       const auto [type, n] = discreteNormal(geometry, x, y, z);
       if (n[0] == 0 && n[1] == 0 && n[2] == 0) return;
-      int kind = OLBGFX_LOCAL_VELOCITY;
+      int kind = flatKind;
       switch (type) {
-      case NormalType::Flat: kind = OLBGFX_LOCAL_VELOCITY; break;
+      case NormalType::Flat: kind = flatKind; break;
       case NormalType::ExternalEdge:
       case NormalType::ExternalCorner: kind = OLBGFX_FIXED_VELOCITY; break;
       case NormalType::InternalEdge: kind = OLBGFX_INNER_EDGE_VELOCITY; break;
@@ -543,6 +546,28 @@ public:
       }
       const CellID c = geometry.getCellId(x, y, z);
       _slotOfCell[c] = (std::uint8_t)face;
+      for (int a = 0; a < 3; ++a) _normals[3 * (std::size_t)c + a] = (std::int8_t)n[a];
+    });
+    _dirty = true;
+  }
+
+  /// InterpolatedPressure on the flat faces of `material`
+  /// (boundary::set<boundary::InterpolatedPressure>,
+  /// boundary/interpolatedPressure3D.h:45-85): BGK faces with fixed density
+  /// (defineRho, or 1) and the normal velocity from the known populations,
+  /// run through the plane finite-difference processor; every other cell
+  /// keeps its dynamics
+  void setInterpolatedPressure(SuperGeometry<T>& geometry, int material)
+  {
+    if (_normals.empty()) {
+      _normals.assign(3 * _ncells, 0);
+    }
+    geometry.forSpatialLocations(2, [&](int x, int y, int z) {
+      if (geometry.get(x, y, z) != material) return;
+      const auto [type, n] = discreteNormal(geometry, x, y, z);
+      if (type != NormalType::Flat || (n[0] == 0 && n[1] == 0 && n[2] == 0)) return;
+      const CellID c = geometry.getCellId(x, y, z);
+      _slotOfCell[c] = (std::uint8_t)slotFor(OLBGFX_INTERPOLATED_PRESSURE);
       for (int a = 0; a < 3; ++a) _normals[3 * (std::size_t)c + a] = (std::int8_t)n[a];
     });
     _dirty = true;
@@ -963,6 +988,12 @@ struct LocalVelocity {
 /// boundary::LocalPressure<T,D,BGKdynamics> (boundary/localPressure3D.h:36-62)
 struct LocalPressure {
 };
+/// boundary::InterpolatedVelocity<T,D,BGKdynamics> (boundary/interpolatedVelocity3D.h:36-100)
+struct InterpolatedVelocity {
+};
+/// boundary::InterpolatedPressure<T,D,BGKdynamics> (boundary/interpolatedPressure3D.h:36-85)
+struct InterpolatedPressure {
+};
 template <typename BC, typename T, typename DESCRIPTOR>
 void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int material)
 {
@@ -970,6 +1001,10 @@ void set(SuperLattice<T, DESCRIPTOR>& lattice, SuperGeometry<T>& geometry, int m
     lattice.setLocalVelocity(geometry, material);
   } else if constexpr (std::is_same_v<BC, LocalPressure>) {
     lattice.setLocalPressure(geometry, material);
+  } else if constexpr (std::is_same_v<BC, InterpolatedVelocity>) {
+    lattice.setLocalVelocity(geometry, material, OLBGFX_INTERPOLATED_VELOCITY);
+  } else if constexpr (std::is_same_v<BC, InterpolatedPressure>) {
+    lattice.setInterpolatedPressure(geometry, material);
   } else {
     lattice.template defineDynamics<BC::template dynamics>(geometry, material);
   }

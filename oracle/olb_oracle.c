@@ -95,6 +95,8 @@ enum {
   KIND_INNER_EDGE = 10,      /* LocalVelocity inner edge: CombinedRLB<BGK, InnerEdgeVelocityTuple3D> */
   KIND_INNER_CORNER = 11,    /* LocalVelocity inner corner: CombinedRLB<BGK, InnerCornerVelocityTuple3D> */
   KIND_LOCAL_PRESSURE = 12,  /* LocalPressure face: CombinedRLB<BGK, RegularizedPressureBoundaryTuple> */
+  KIND_INTERP_VELOCITY = 13, /* InterpolatedVelocity face: BGK<BasicDirichletVelocityBoundaryTuple> + PlaneFd */
+  KIND_INTERP_PRESSURE = 14, /* InterpolatedPressure face: BGK<BasicDirichletPressureBoundaryTuple> + PlaneFd */
 };
 
 enum { STORE_NATIVE = 0, STORE_F16 = 1, STORE_BF16 = 2 };
@@ -753,6 +755,24 @@ static int collide_cell(const lattice_t* L, uint64_t n, real f[Q], real* st_rho,
     *st_rho = rho; *st_usqr = norm_sqr(uu);
     return 1;
   }
+  case KIND_INTERP_VELOCITY:
+  case KIND_INTERP_PRESSURE: {
+    /* BGK (collision.h:57-80) through BasicDirichletVelocityBoundaryTuple or
+     * BasicDirichletPressureBoundaryTuple (momenta/aliases.h:173-198) */
+    int dir = 0, orient = 0;
+    if (!flat_normal(L, n, &dir, &orient)) return 0;
+    const real* uw = wall_velocity(L, D, n);
+    real uu[3] = {uw[0], uw[1], uw[2]};
+    if (D->kind == KIND_INTERP_VELOCITY) {
+      rho = velocity_bm_rho(f, dir, orient, uu);
+    } else {
+      rho = wall_density(L, D, n);
+      pressure_u(f, dir, orient, rho, uw, uu);
+    }
+    bgk(f, rho, uu, D->omega);
+    *st_rho = rho; *st_usqr = norm_sqr(uu);
+    return 1;
+  }
   case KIND_FIXED_VELOCITY: {
     /* BulkDensity + FixedVelocityMomentumGeneric, BGK */
     const real* uw = wall_velocity(L, D, n);
@@ -834,7 +854,8 @@ static void periodic_copy(lattice_t* L)
 /* PostStream overlap exchange of the post-processor neighbourhood
  * (boundary/setBoundary3D.h:57-110 addPoints2CommBC requests the populations
  * of the padding cells near the boundary): the padding cells within reach
- * (2) of an edge or corner processor -- the ones those processors read --
+ * (2) of an edge, corner or plane finite-difference processor -- the ones
+ * those processors read --
  * whose periodic image lies in the core get the image's streamed
  * populations, before any post-processor runs. */
 static int near_processor(const lattice_t* L, const int g[3])
@@ -846,7 +867,9 @@ static int near_processor(const lattice_t* L, const int g[3])
         const int c[3] = {g[0] + dx, g[1] + dy, g[2] + dz};
         if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= L->nx || c[1] >= L->ny || c[2] >= L->nz) continue;
         const uint64_t n = (uint64_t)((c[0] + L->pad) * L->proj[0] + (c[1] + L->pad) * L->proj[1] + (c[2] + L->pad));
-        if (L->table[L->mask[n]].kind != KIND_FIXED_VELOCITY) continue;
+        const int kind = L->table[L->mask[n]].kind;
+        if (kind == KIND_INTERP_VELOCITY || kind == KIND_INTERP_PRESSURE) return 1;
+        if (kind != KIND_FIXED_VELOCITY) continue;
         const int nz = (L->normals[3 * n] != 0) + (L->normals[3 * n + 1] != 0) + (L->normals[3 * n + 2] != 0);
         if (nz >= 2) return 1;
       }
@@ -914,8 +937,10 @@ static real momenta_rho(const lattice_t* L, uint64_t n)
   case KIND_BOUNCE_BACK:
   case KIND_BOUNCE_BACK_VELOCITY:
   case KIND_LOCAL_PRESSURE:
+  case KIND_INTERP_PRESSURE:
     return wall_density(L, D, n);        /* FixedDensity */
-  case KIND_LOCAL_VELOCITY: {
+  case KIND_LOCAL_VELOCITY:
+  case KIND_INTERP_VELOCITY: {
     int dir = 0, orient = 0;
     flat_normal(L, n, &dir, &orient);
     const real* uw = wall_velocity(L, D, n);
@@ -949,12 +974,14 @@ static void momenta_u(const lattice_t* L, uint64_t n, real u[3])
   case KIND_FIXED_VELOCITY:
   case KIND_INNER_EDGE:
   case KIND_INNER_CORNER:
-  case KIND_BOUNCE_BACK_VELOCITY: {
+  case KIND_BOUNCE_BACK_VELOCITY:
+  case KIND_INTERP_VELOCITY: {
     const real* uw = wall_velocity(L, D, n);
     for (int a = 0; a < 3; ++a) u[a] = uw[a];
     return;
   }
-  case KIND_LOCAL_PRESSURE: {
+  case KIND_LOCAL_PRESSURE:
+  case KIND_INTERP_PRESSURE: {
     real f[Q];
     int dir = 0, orient = 0;
     load_cell(L, n, f);
@@ -1053,6 +1080,28 @@ static void post_corner(lattice_t* L, uint64_t n, const int N[3])
   regularise(L, n, rho, du);
 }
 
+/* PlaneFdBoundaryProcessor3D<direction, orientation>::apply
+ * (boundary/boundaryPostProcessors3D.hh:39-92): density and velocity
+ * through the momenta, the gradient one-sided along the normal and central
+ * along the wall (finiteDifference3D.h:75-230) */
+static void post_plane_fd(lattice_t* L, uint64_t n)
+{
+  int dir = 0, orient = 0;
+  flat_normal(L, n, &dir, &orient);
+  const real rho = momenta_rho(L, n);
+  real du[3][3], up[3], um[3];
+  for (int a = 0; a < 3; ++a) {
+    if (a == dir) {
+      one_sided(L, n, a, orient, du[a]);
+    } else {
+      momenta_u(L, step_cell(L, n, a, 1), up);
+      momenta_u(L, step_cell(L, n, a, -1), um);
+      for (int b = 0; b < 3; ++b) du[a][b] = (up[b] - um[b]) / (real)2;
+    }
+  }
+  regularise(L, n, rho, du);
+}
+
 /* BouzidiPostProcessor::apply (boundary/setBouzidiBoundary.h:48-80) on every
  * listed cell in the listed order (the reference's ascending cell id,
  * core/platform/cpu/sisd/operator.h:368-386), in place.  Neighbours are read
@@ -1083,17 +1132,23 @@ static void post_bouzidi(lattice_t* L)
 }
 
 /* PostStream stage: Bouzidi (priority -1), LocalVelocity edges (priority 0),
- * corners (priority 1) */
+ * corners (priority 1), then the flat faces of the interpolated boundaries
+ * (priority 0), each in ascending cell id and in place */
 static void post_stream(lattice_t* L)
 {
   post_bouzidi(L);
   if (!L->normals) return;
-  for (int pass = 2; pass <= 3; ++pass) {
+  for (int pass = 2; pass <= 4; ++pass) {
     for (int x = 0; x < L->nx; ++x) {
       for (int y = 0; y < L->ny; ++y) {
         for (int z = 0; z < L->nz; ++z) {
           const uint64_t n = (uint64_t)((x + L->pad) * L->proj[0] + (y + L->pad) * L->proj[1] + (z + L->pad));
-          if (L->table[L->mask[n]].kind != KIND_FIXED_VELOCITY) continue;
+          const int kind = L->table[L->mask[n]].kind;
+          if (pass == 4) {
+            if (kind == KIND_INTERP_VELOCITY || kind == KIND_INTERP_PRESSURE) post_plane_fd(L, n);
+            continue;
+          }
+          if (kind != KIND_FIXED_VELOCITY) continue;
           const int N[3] = {L->normals[3 * n], L->normals[3 * n + 1], L->normals[3 * n + 2]};
           const int nz = (N[0] != 0) + (N[1] != 0) + (N[2] != 0);
           if (nz != pass) continue;

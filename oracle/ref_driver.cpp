@@ -26,6 +26,9 @@
  *                                                stops after that many steps instead
  *   olb_ref poiseuille_local_nonforced N OUT      its non-forced local row (LocalVelocity wall and
  *                                                inflow, LocalPressure outflow)
+ *   olb_ref poiseuille_interpolated N OUT [STEPS]  its forced interpolated row (InterpolatedVelocity
+ *                                                wall); _nonforced: InterpolatedVelocity wall and
+ *                                                inflow, InterpolatedPressure outflow
  *   olb_ref normals IN OUT                       discrete normals (boundary::set's classifier)
  *                                                of materials 2 and 3 on the geometry in IN
  *   olb_ref lv_geometry IN STEPS ULID RE OUT     LocalVelocity walls (2) and lid (3) on the
@@ -563,7 +566,8 @@ int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir
 // and energies, the discrete normals of materials 2-4, every cell's
 // computeRhoU after initialize (the velocity and density the boundary momenta
 // hold) and the analytic velocity and pressure of the fluid.
-int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSteps = -1)
+int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSteps = -1,
+                         bool interpolated = false)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
   const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
@@ -611,12 +615,22 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
     L.template defineDynamics<BGKdynamics>(G, 1);
     L.template defineDynamics<BGKdynamics>(G, 2);
   }
-  boundary::set<boundary::LocalVelocity>(L, G, 2);
+  // the interpolated row (testPoiseuille3DBGK.cpp:158-160, 183-197)
+  if (interpolated) {
+    boundary::set<boundary::InterpolatedVelocity>(L, G, 2);
+  } else {
+    boundary::set<boundary::LocalVelocity>(L, G, 2);
+  }
   if (!forced) {
     L.template defineDynamics<BGKdynamics>(G, 3);
-    boundary::set<boundary::LocalVelocity>(L, G, 3);
     L.template defineDynamics<BGKdynamics>(G, 4);
-    boundary::set<boundary::LocalPressure>(L, G, 4);
+    if (interpolated) {
+      boundary::set<boundary::InterpolatedVelocity>(L, G, 3);
+      boundary::set<boundary::InterpolatedPressure>(L, G, 4);
+    } else {
+      boundary::set<boundary::LocalVelocity>(L, G, 3);
+      boundary::set<boundary::LocalPressure>(L, G, 4);
+    }
   }
   std::vector<T> axisPoint = {lx, r, r};
   std::vector<T> axisDirection = {1, 0, 0};
@@ -721,9 +735,9 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   }
   char buf[768];
   std::snprintf(buf, sizeof(buf),
-                " case=poiseuille boundary=local forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
+                " case=poiseuille boundary=%s forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
                 "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu",
-                forced ? 1 : 0, n, iT, (double)omega, (double)force0,
+                interpolated ? "interpolated" : "local", forced ? 1 : 0, n, iT, (double)omega, (double)force0,
                 (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
                 converter.getLatticeTime(maxPhysT));
@@ -912,6 +926,11 @@ int main(int argc, char** argv)
     if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && (argc == 4 || argc == 5)) {
       return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local",
                                   argc == 5 ? std::stoi(argv[4]) : -1);
+    }
+    if ((mode == "poiseuille_interpolated" || mode == "poiseuille_interpolated_nonforced") &&
+        (argc == 4 || argc == 5)) {
+      return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_interpolated",
+                                  argc == 5 ? std::stoi(argv[4]) : -1, true);
     }
     if (mode == "lv_geometry" && argc == 7) {
       return run_lv_geometry(argv[2], std::stoi(argv[3]), std::stod(argv[4]), std::stod(argv[5]), argv[6]);

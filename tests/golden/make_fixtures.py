@@ -235,6 +235,8 @@ def main() -> None:
                 vtk_fixtures()
             elif name == "poiseuille_local":
                 poiseuille_local()
+            elif name == "poiseuille_interpolated":
+                poiseuille_interpolated()
             elif name == "normals":
                 normals_fixture()
             elif name == "step_cavity":
@@ -320,10 +322,26 @@ def poiseuille_local() -> None:
     """testPoiseuille3DBGK's local rows at N = 11, FP64, run to convergence by
     the reference: forced (LocalVelocity wall, periodic) and non-forced
     (LocalVelocity wall and inflow, LocalPressure outflow)"""
-    for mode, name in (("poiseuille_local", "poiseuille11_forced_local_f64"),
-                       ("poiseuille_local_nonforced", "poiseuille11_nonforced_local_f64")):
+    poiseuille_rows((("poiseuille_local", "poiseuille11_forced_local_f64"),
+                     ("poiseuille_local_nonforced", "poiseuille11_nonforced_local_f64")))
+
+
+def poiseuille_interpolated() -> None:
+    """testPoiseuille3DBGK's interpolated rows at N = 11, FP64, run to
+    convergence by the reference: forced (InterpolatedVelocity wall,
+    periodic) and non-forced (InterpolatedVelocity wall and inflow,
+    InterpolatedPressure outflow)"""
+    # one thread: the reference applies PlaneFdBoundaryProcessor3D under
+    # omp parallel for, and a face cell reads the velocity of its neighbours
+    # on the outlet from populations another thread may be rewriting
+    poiseuille_rows((("poiseuille_interpolated", "poiseuille11_forced_interpolated_f64"),
+                     ("poiseuille_interpolated_nonforced", "poiseuille11_nonforced_interpolated_f64")), threads=1)
+
+
+def poiseuille_rows(rows, threads: int = 4) -> None:
+    for mode, name in rows:
         path = os.path.join(RUN, mode + ".bin")
-        run("olb_ref_d", [mode, "11", path])
+        run("olb_ref_d", [mode, "11", path], threads=threads)
         meta, arrays = read_dump(path)
         mats = arrays["materials"]
         keep = np.flatnonzero(mats != 0)

@@ -59,15 +59,20 @@ def dynamics_for(meta: dict, storage: str) -> dict:
                 2: Dynamics(BOUNCE_BACK)}
     if case == "poiseuille":
         forced = Dynamics(FORCED_BGK, omega=meta["omega"], force=(meta["force"], 0.0, 0.0))
-        if meta.get("boundary") == "local":
+        if meta.get("boundary") in ("local", "interpolated"):
             # testPoiseuille3DBGK.cpp:118-199: the bulk dynamics on materials 1-4,
-            # then LocalVelocity (2, 3) and LocalPressure (4) on top
+            # then LocalVelocity (2, 3) and LocalPressure (4) on top, or
+            # InterpolatedVelocity and InterpolatedPressure
+            from openlb_amd.cases import InterpolatedPressure, InterpolatedVelocity
+
             bulk = forced if meta["forced"] else Dynamics(BGK, omega=meta["omega"])
+            vel, pre = ((InterpolatedVelocity, InterpolatedPressure) if meta["boundary"] == "interpolated"
+                        else (LocalVelocity, LocalPressure))
             if meta["forced"]:
-                return {1: bulk, 2: LocalVelocity(omega=meta["omega"], base=bulk)}
-            return {1: bulk, 2: LocalVelocity(omega=meta["omega"], base=bulk),
-                    3: LocalVelocity(omega=meta["omega"], base=bulk),
-                    4: LocalPressure(omega=meta["omega"], base=bulk)}
+                return {1: bulk, 2: vel(omega=meta["omega"], base=bulk)}
+            return {1: bulk, 2: vel(omega=meta["omega"], base=bulk),
+                    3: vel(omega=meta["omega"], base=bulk),
+                    4: pre(omega=meta["omega"], base=bulk)}
         wall = Dynamics(NO_DYNAMICS) if meta.get("boundary") == "bouzidi" else Dynamics(BOUNCE_BACK)
         return {1: forced, 2: wall}
     raise ValueError(case)
@@ -89,16 +94,11 @@ def build(lattice_cls, meta, d, storage: str):
     slot = 1
     lv = {}
     for mat, dyn in sorted(dyns.items()):
-        if isinstance(dyn, (LocalVelocity, LocalPressure)):
+        if hasattr(dyn, "shapes"):
             # one slot per boundary shape; the material's base dynamics for
             # cells boundary::set gave no normal
-            if isinstance(dyn, LocalVelocity):
-                shapes = {FLAT: LOCAL_VELOCITY, EXTERNAL_EDGE: FIXED_VELOCITY, EXTERNAL_CORNER: FIXED_VELOCITY,
-                          INTERNAL_EDGE: INNER_EDGE_VELOCITY, INTERNAL_CORNER: INNER_CORNER_VELOCITY}
-                extra = {"wall_velocity": dyn.wall_velocity}
-            else:
-                shapes = {FLAT: LOCAL_PRESSURE}
-                extra = {"wall_rho": dyn.rho}
+            shapes = dict(dyn.shapes())
+            extra = dyn.extra()
             by_type = {}
             kinds = {}
             for t, kind in shapes.items():
@@ -120,10 +120,11 @@ def build(lattice_cls, meta, d, storage: str):
     if lv:
         # the reference's own discrete normals of the boundary cells
         types = d["normal_type"]
+        has_normal = d["normals"].reshape(-1, 3).any(axis=1)
         for mat, by_type in lv.items():
             on = mats == mat
             for t, sl in by_type.items():
-                mask[on & (types == t)] = sl
+                mask[on & (types == t) & (has_normal | (t != FLAT))] = sl
         lat.set_cell_normals(d["normals"])
     lat.set_dynamics_mask(mask)
     if "u0" in d:

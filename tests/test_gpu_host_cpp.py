@@ -114,18 +114,19 @@ def test_cpp_cavity_vtk_matches_reference(tmp_path, boundary, fixture):
     assert head((tmp_path / "data" / (vtk.file_name(name, 20, 0) + ".vti")).read_text()) == head(str(d["vti"]))
 
 
-def test_cpp_channel_matches_oracle(tmp_path):
-    """the C++ mirror's channel (BounceBack walls, LocalVelocity inlet,
-    LocalPressure outlet; boundary::set<boundary::LocalPressure> and
-    defineRho) against the oracle on the same case after 150 steps (FP32,
-    1e-5 relative on the fluid and the inlet and outlet faces)"""
+@pytest.mark.parametrize("boundary", ["local", "interpolated"])
+def test_cpp_channel_matches_oracle(tmp_path, boundary):
+    """the C++ mirror's channel (BounceBack walls, LocalVelocity inlet and
+    LocalPressure outlet with defineRho, or InterpolatedVelocity and
+    InterpolatedPressure) against the oracle on the same case after 150
+    steps (FP32, 1e-5 relative on the fluid)"""
     binary = os.path.join(ROOT, "openlb_amd", "host", "examples", "channel3d")
     dump = tmp_path / "channel.bin"
-    print("[cpp] channel3d", flush=True)
+    print("[cpp] channel3d", boundary, flush=True)
     out = subprocess.run([binary, "--NX", "24", "--NY", "10", "--NZ", "10", "--TIME_STEPS", "150",
-                          "--DUMP", str(dump)], capture_output=True, text=True, timeout=120)
+                          "--DUMP", str(dump), "--BOUNDARY", boundary], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    c = cases.channel3d(24, 10, 10, u_in=0.05, tau=0.8, dtype=np.float32)
+    c = cases.channel3d(24, 10, 10, u_in=0.05, tau=0.8, dtype=np.float32, boundary=boundary)
     raw = np.fromfile(dump, dtype=np.uint8)
     np.testing.assert_array_equal(raw[:c.ncells], c.materials.reshape(-1))
     f = raw[c.ncells:].view(np.float32).reshape(19, c.ncells)
