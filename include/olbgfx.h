@@ -184,6 +184,18 @@ olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* lattice, const int8_t* nor
 olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* lattice, int64_t n, const int64_t* cells,
                                  int32_t value_type, const void* q);
 
+/* the cells that run BouzidiVelocityPostProcessor (setBouzidiBoundary.h:
+ * 129-164, the inflow of the Poiseuille KAT's non-forced Bouzidi rows) with
+ * their BOUZIDI_DISTANCE and BOUZIDI_VELOCITY fields (19 values each per
+ * cell, the coefficient c_i . u_wall that setBouzidiVelocity writes,
+ * setBouzidiBoundary.h:382-420): the Bouzidi update of olbgfx_set_bouzidi
+ * minus the wall-velocity term (2 or 1/q) t_i coefficient / cs^2.  Applied
+ * after the olbgfx_set_bouzidi list each step (same priority, later in the
+ * reference's operator-type order); replaces an earlier list, n = 0
+ * removes it.  Same rules for the cells as olbgfx_set_bouzidi. */
+olbgfx_status olbgfx_set_bouzidi_velocity(olbgfx_lattice* lattice, int64_t n, const int64_t* cells,
+                                          int32_t value_type, const void* q, const void* velocity);
+
 /* ---- populations -----------------------------------------------------------/* ---- populations -----------------------------------------------------------
  * iniEquilibrium(indicator, rho, u) (core/blockLattice.hh:202-236,
  * dynamics/interface.h:111-121): f_bar_i = feq_i(rho, u) on selected cells.

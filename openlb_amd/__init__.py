@@ -105,6 +105,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_set_cell_field": [vp, i32, i32, vp],
         "olbgfx_set_cell_normals": [vp, vp],
         "olbgfx_set_bouzidi": [vp, ctypes.c_int64, vp, i32, vp],
+        "olbgfx_set_bouzidi_velocity": [vp, ctypes.c_int64, vp, i32, vp, vp],
         "olbgfx_post_process": [vp, vp],
         "olbgfx_ini_equilibrium": [vp, vp, i32, vp, vp],
         "olbgfx_get_populations": [vp, i32, vp],
@@ -302,6 +303,19 @@ class Lattice:
         if q.size != 19 * cells.size:
             raise ValueError("q must hold 19 distances per cell")
         _check(library().olbgfx_set_bouzidi(self._h, int(cells.size), _ptr(cells), VALUE_F64, _ptr(q)))
+
+    def set_bouzidi_velocity(self, cells, q, velocity) -> None:
+        """setBouzidiBoundary<BouzidiVelocityPostProcessor> with
+        setBouzidiVelocity: the core cells (ascending ids), their
+        BOUZIDI_DISTANCE and BOUZIDI_VELOCITY coefficients (n x 19 each),
+        applied after the set_bouzidi list.  An empty list removes them."""
+        cells = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
+        if q.size != 19 * cells.size or v.size != 19 * cells.size:
+            raise ValueError("q and velocity must hold 19 values per cell")
+        _check(library().olbgfx_set_bouzidi_velocity(self._h, int(cells.size), _ptr(cells), VALUE_F64, _ptr(q),
+                                                     _ptr(v)))
 
     def ini_equilibrium(self, rho: np.ndarray, u: np.ndarray, sel: np.ndarray | None = None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)

@@ -1267,7 +1267,8 @@ __global__ void __launch_bounds__(256) k_refresh(const StepArgs<typename Storage
 // cell's update writes, so the cells can be updated concurrently.
 template <typename S>
 __global__ void __launch_bounds__(256) k_bouzidi(const StepArgs<typename Storage<S>::A> a, const uint32_t* cells,
-                                                 const typename Storage<S>::A* q, uint32_t ncells)
+                                                 const typename Storage<S>::A* q, const typename Storage<S>::A* vel,
+                                                 uint32_t ncells)
 {
   using A = typename Storage<S>::A;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1284,20 +1285,23 @@ __global__ void __launch_bounds__(256) k_bouzidi(const StepArgs<typename Storage
     constexpr int half = 9;
     const int o = i <= half ? i + half : i - half;
     const A qi = q[(size_t)(i - 1) * ncells + t];
+    // BouzidiVelocityPostProcessor (setBouzidiBoundary.h:129-164): the wall
+    // velocity term BOUZIDI_VELOCITY_i t_i / cs^2
+    const A vt = vel ? vel[(size_t)(i - 1) * ncells + t] * (A(1) / A(TDEN(i))) * A(3) : A(0);
     if (qi > A(0)) {
       const int64_t d = CX(i) * step[0] + CY(i) * step[1] + CZ(i) * step[2];
       const A xs = Storage<S>::load(ring_at<S>(a.pop, a.stride, i, ring_slot((uint32_t)((int64_t)n + d),
                                                                               a.shift[i], a.count)));
       if (qi <= A(0.5)) {
-        f[o] = A(2) * qi * xs + (A(1) - A(2) * qi) * f[i];
+        f[o] = A(2) * qi * xs + (A(1) - A(2) * qi) * f[i] - A(2) * vt;
       } else {
         const A xf = Storage<S>::load(ring_at<S>(a.pop, a.stride, o, ring_slot((uint32_t)((int64_t)n - d),
                                                                                 a.shift[o], a.count)));
-        f[o] = A(0.5) / qi * xs + A(0.5) * (A(2) * qi - A(1)) / qi * xf;
+        f[o] = A(0.5) / qi * xs + A(0.5) * (A(2) * qi - A(1)) / qi * xf - A(1) / qi * vt;
       }
       written |= 1u << o;
     } else if (qi == A(0)) {
-      f[o] = f[i];
+      f[o] = f[i] - A(2) * vt;
       written |= 1u << o;
     }
   }
@@ -1654,6 +1658,13 @@ struct olbgfx_lattice {
   uint32_t* bz_cells = nullptr;
   void* bz_q = nullptr;
   uint32_t bz_n = 0;
+  // BouzidiVelocityPostProcessor cells: distances as above and the
+  // BOUZIDI_VELOCITY coefficients in the same layout; applied after the
+  // plain list (same priority, the reference's operator-type order)
+  uint32_t* bzv_cells = nullptr;
+  void* bzv_q = nullptr;
+  void* bzv_v = nullptr;
+  uint32_t bzv_n = 0;
   uint64_t boundary_version = ~0ull;
   olbgfx_dynamics_desc table[OLBGFX_MAX_DYNAMICS];
   std::vector<uint8_t> host_mask;
@@ -1779,7 +1790,13 @@ olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a
     HIP_TRY(hipGetLastError());
   }
   if (L->bz_n > 0) {
-    k_bouzidi<S><<<(L->bz_n + 255) / 256, 256, 0, stream>>>(a, L->bz_cells, (const A*)L->bz_q, L->bz_n);
+    k_bouzidi<S><<<(L->bz_n + 255) / 256, 256, 0, stream>>>(a, L->bz_cells, (const A*)L->bz_q, nullptr,
+                                                           L->bz_n);
+    HIP_TRY(hipGetLastError());
+  }
+  if (L->bzv_n > 0) {
+    k_bouzidi<S><<<(L->bzv_n + 255) / 256, 256, 0, stream>>>(a, L->bzv_cells, (const A*)L->bzv_q,
+                                                            (const A*)L->bzv_v, L->bzv_n);
     HIP_TRY(hipGetLastError());
   }
   if (L->post_edges) {
@@ -2256,6 +2273,9 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->refresh_dev);
   hipFree(L->bz_cells);
   hipFree(L->bz_q);
+  hipFree(L->bzv_cells);
+  hipFree(L->bzv_q);
+  hipFree(L->bzv_v);
   hipFree(L->stats_partial);
   hipFree(L->stats_out);
   delete L;
@@ -2365,8 +2385,14 @@ olbgfx_status olbgfx_set_cell_normals(olbgfx_lattice* L, const int8_t* normals)
   return OLBGFX_OK;
 }
 
-olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* L, int64_t n, const int64_t* cells, int32_t vt,
-                                 const void* q)
+}  // extern "C"
+
+namespace {
+
+// one Bouzidi list: validated, then uploaded direction-major; vel (may be
+// null) holds the BOUZIDI_VELOCITY coefficients of the velocity variant
+olbgfx_status set_bouzidi_list(olbgfx_lattice* L, int64_t n, const int64_t* cells, int32_t vt, const void* q,
+                               const void* vel, uint32_t*& dev_cells, void*& dev_q, void*& dev_v, uint32_t& dev_n)
 {
   if (check(L) != OLBGFX_OK) {
     return OLBGFX_ERR_INVALID;
@@ -2387,6 +2413,13 @@ olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* L, int64_t n, const int64_t* ce
     qd[k] = vt == OLBGFX_VALUE_F64 ? ((const double*)q)[k] : (double)((const float*)q)[k];
     if (!std::isfinite(qd[k])) {
       return fail(OLBGFX_ERR_INVALID, "Bouzidi distances must be finite");
+    }
+  }
+  std::vector<double> vd(vel ? Q * (size_t)n : 0);
+  for (size_t k = 0; k < vd.size(); ++k) {
+    vd[k] = vt == OLBGFX_VALUE_F64 ? ((const double*)vel)[k] : (double)((const float*)vel)[k];
+    if (!std::isfinite(vd[k])) {
+      return fail(OLBGFX_ERR_INVALID, "Bouzidi velocity coefficients must be finite");
     }
   }
   for (int64_t k = 0; k < n; ++k) {
@@ -2428,35 +2461,73 @@ olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* L, int64_t n, const int64_t* ce
   }
   ++L->version;
   HIP_TRY(hipDeviceSynchronize());
-  HIP_TRY(hipFree(L->bz_cells));
-  HIP_TRY(hipFree(L->bz_q));
-  L->bz_cells = nullptr;
-  L->bz_q = nullptr;
-  L->bz_n = 0;
+  HIP_TRY(hipFree(dev_cells));
+  HIP_TRY(hipFree(dev_q));
+  HIP_TRY(hipFree(dev_v));
+  dev_cells = nullptr;
+  dev_q = nullptr;
+  dev_v = nullptr;
+  dev_n = 0;
   if (n == 0) {
     return OLBGFX_OK;
   }
   std::vector<uint32_t> ids(cells, cells + n);
   // direction-major on the device so the threads of a wavefront read q coalesced
   const size_t as = is_f64(L) ? 8 : 4;
-  std::vector<char> qa(as * 18 * (size_t)n);
-  for (int64_t k = 0; k < n; ++k) {
-    for (int i = 1; i < Q; ++i) {
-      const size_t at = (size_t)(i - 1) * n + k;
-      if (as == 8) {
-        ((double*)qa.data())[at] = qd[Q * k + i];
-      } else {
-        ((float*)qa.data())[at] = (float)qd[Q * k + i];
+  auto direction_major = [&](const std::vector<double>& src) {
+    std::vector<char> out(as * 18 * (size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+      for (int i = 1; i < Q; ++i) {
+        const size_t at = (size_t)(i - 1) * n + k;
+        if (as == 8) {
+          ((double*)out.data())[at] = src[Q * k + i];
+        } else {
+          ((float*)out.data())[at] = (float)src[Q * k + i];
+        }
       }
     }
+    return out;
+  };
+  const std::vector<char> qa = direction_major(qd);
+  HIP_TRY(hipMalloc(&dev_cells, sizeof(uint32_t) * ids.size()));
+  HIP_TRY(hipMalloc(&dev_q, qa.size()));
+  HIP_TRY(hipMemcpy(dev_cells, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dev_q, qa.data(), qa.size(), hipMemcpyHostToDevice));
+  if (vel) {
+    const std::vector<char> va = direction_major(vd);
+    HIP_TRY(hipMalloc(&dev_v, va.size()));
+    HIP_TRY(hipMemcpy(dev_v, va.data(), va.size(), hipMemcpyHostToDevice));
   }
-  HIP_TRY(hipMalloc(&L->bz_cells, sizeof(uint32_t) * ids.size()));
-  HIP_TRY(hipMalloc(&L->bz_q, qa.size()));
-  HIP_TRY(hipMemcpy(L->bz_cells, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(L->bz_q, qa.data(), qa.size(), hipMemcpyHostToDevice));
-  L->bz_n = (uint32_t)n;
+  dev_n = (uint32_t)n;
   return OLBGFX_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* L, int64_t n, const int64_t* cells, int32_t vt, const void* q)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  void* unused = nullptr;
+  return set_bouzidi_list(L, n, cells, vt, q, nullptr, L->bz_cells, L->bz_q, unused, L->bz_n);
+}
+
+olbgfx_status olbgfx_set_bouzidi_velocity(olbgfx_lattice* L, int64_t n, const int64_t* cells, int32_t vt,
+                                          const void* q, const void* velocity)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (n > 0 && !velocity) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  return set_bouzidi_list(L, n, cells, vt, q, velocity, L->bzv_cells, L->bzv_q, L->bzv_v, L->bzv_n);
+}
+
+
 
 olbgfx_status olbgfx_set_cell_field(olbgfx_lattice* L, int32_t field, int32_t vt, const void* values)
 {
@@ -2842,7 +2913,7 @@ olbgfx_status ensemble_prepare(olbgfx_ensemble* E, bool force)
     if (L->boundary_kinds) {
       return fail(OLBGFX_ERR_UNSUPPORTED, "ensembles do not take members with LocalVelocity boundaries");
     }
-    if (L->bz_n > 0) {
+    if (L->bz_n > 0 || L->bzv_n > 0) {
       return fail(OLBGFX_ERR_UNSUPPORTED, "ensembles do not take members with Bouzidi boundaries");
     }
   }

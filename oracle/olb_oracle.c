@@ -144,6 +144,10 @@ typedef struct {
   int64_t bz_n;         /* cells running BouzidiPostProcessor */
   int64_t* bz_cells;    /* their ids, in the order they are applied */
   real* bz_q;           /* their BOUZIDI_DISTANCE, 19 per cell */
+  int64_t bzv_n;        /* cells running BouzidiVelocityPostProcessor */
+  int64_t* bzv_cells;
+  real* bzv_q;
+  real* bzv_v;          /* their BOUZIDI_VELOCITY, 19 per cell */
   int store;
   int stats_on;
   double stats[4];      /* nCells, sum rho, sum uSqr, max uSqr (last step) */
@@ -294,6 +298,9 @@ void ORC(destroy)(void* h)
   free(L->normals);
   free(L->bz_cells);
   free(L->bz_q);
+  free(L->bzv_cells);
+  free(L->bzv_q);
+  free(L->bzv_v);
   free(L);
 }
 
@@ -1108,27 +1115,38 @@ static void post_plane_fd(lattice_t* L, uint64_t n)
  * from their own ring slots: the values read (x_s[i], x_f[opp i] and x_b[i])
  * all streamed from a core cell, so a padding cell's slot already holds what
  * the reference's PostStream overlap exchange would put there. */
-static void post_bouzidi(lattice_t* L)
+static void bouzidi_list(lattice_t* L, int64_t count, const int64_t* cells, const real* qs, const real* vs)
 {
-  for (int64_t e = 0; e < L->bz_n; ++e) {
-    const uint64_t n = (uint64_t)L->bz_cells[e];
-    const real* q = &L->bz_q[Q * e];
+  for (int64_t e = 0; e < count; ++e) {
+    const uint64_t n = (uint64_t)cells[e];
+    const real* q = &qs[Q * e];
     for (int i = 1; i < Q; ++i) {
       const int o = opp(i);
       real* xb_o = &L->ring[o][slot(L, o, n)];
       const real xb_i = L->ring[i][slot(L, i, n)];
+      /* BouzidiVelocityPostProcessor's wall term (setBouzidiBoundary.h:149) */
+      const real vt = vs ? vs[Q * e + i] * weight(i) * (real)3 : (real)0;
       if (q[i] > (real)0) {
         const int64_t dn = L->d[i];
         const real xs = L->ring[i][slot(L, i, (uint64_t)((int64_t)n + dn))];
         const real xf = L->ring[o][slot(L, o, (uint64_t)((int64_t)n - dn))];
         const real lo = (real)(q[i] <= (real)0.5), hi = (real)(q[i] > (real)0.5);
-        *xb_o = orc_quantize(L, lo * ((real)2 * q[i] * xs + ((real)1 - (real)2 * q[i]) * xb_i)
-                              + hi * ((real)0.5 / q[i] * xs + (real)0.5 * ((real)2 * q[i] - (real)1) / q[i] * xf));
+        *xb_o = orc_quantize(L, lo * ((real)2 * q[i] * xs + ((real)1 - (real)2 * q[i]) * xb_i - (real)2 * vt)
+                              + hi * ((real)0.5 / q[i] * xs + (real)0.5 * ((real)2 * q[i] - (real)1) / q[i] * xf
+                                      - (real)1 / q[i] * vt));
       } else if (q[i] == (real)0) {
-        *xb_o = xb_i;
+        *xb_o = orc_quantize(L, xb_i - (real)2 * vt);
       }
     }
   }
+}
+
+/* BouzidiPostProcessor, then BouzidiVelocityPostProcessor (priority -1,
+ * in the reference's operator-type order) */
+static void post_bouzidi(lattice_t* L)
+{
+  bouzidi_list(L, L->bz_n, L->bz_cells, L->bz_q, NULL);
+  bouzidi_list(L, L->bzv_n, L->bzv_cells, L->bzv_q, L->bzv_v);
 }
 
 /* PostStream stage: Bouzidi (priority -1), LocalVelocity edges (priority 0),
@@ -1188,6 +1206,28 @@ void ORC(set_bouzidi)(void* h, int64_t n, const int64_t* cells, const real* q)
   L->bz_n = n;
 }
 
+/* the cells running BouzidiVelocityPostProcessor with BOUZIDI_DISTANCE and
+ * BOUZIDI_VELOCITY (19 each per cell); n = 0 removes them */
+void ORC(set_bouzidi_velocity)(void* h, int64_t n, const int64_t* cells, const real* q, const real* v)
+{
+  lattice_t* L = (lattice_t*)h;
+  free(L->bzv_cells);
+  free(L->bzv_q);
+  free(L->bzv_v);
+  L->bzv_cells = NULL;
+  L->bzv_q = NULL;
+  L->bzv_v = NULL;
+  L->bzv_n = 0;
+  if (n <= 0) return;
+  L->bzv_cells = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+  L->bzv_q = (real*)malloc(sizeof(real) * Q * (size_t)n);
+  L->bzv_v = (real*)malloc(sizeof(real) * Q * (size_t)n);
+  memcpy(L->bzv_cells, cells, sizeof(int64_t) * (size_t)n);
+  memcpy(L->bzv_q, q, sizeof(real) * Q * (size_t)n);
+  memcpy(L->bzv_v, v, sizeof(real) * Q * (size_t)n);
+  L->bzv_n = n;
+}
+
 /* BlockLattice::postProcess() (core/blockLattice.hh:56-67): the PostStream
  * processors on the current populations, as initialize() runs them once */
 void ORC(post_process)(void* h)
@@ -1202,7 +1242,7 @@ void ORC(collide_and_stream)(void* h, int steps)
     collide(L);
     periodic_copy(L);
     stream(L);
-    if (L->bz_n > 0 || L->normals) {
+    if (L->bz_n > 0 || L->bzv_n > 0 || L->normals) {
       periodic_refresh(L);
     }
     post_stream(L);

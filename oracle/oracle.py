@@ -51,6 +51,7 @@ def library() -> ctypes.CDLL:
             g("set_cell_force").argtypes = [vp, vp]
             g("set_cell_normals").argtypes = [vp, vp]
             g("set_bouzidi").argtypes = [vp, ctypes.c_int64, vp, vp]
+            g("set_bouzidi_velocity").argtypes = [vp, ctypes.c_int64, vp, vp, vp]
             g("post_process").argtypes = [vp]
             g("get_populations").argtypes = [vp, vp]
             g("set_populations").argtypes = [vp, vp]
@@ -171,6 +172,16 @@ class OracleLattice:
         q = np.ascontiguousarray(q, dtype=self._dtype).reshape(-1)
         assert q.size == Q * cells.size
         self._f("set_bouzidi")(self._h, int(cells.size), _ptr(cells), _ptr(q))
+
+    def set_bouzidi_velocity(self, cells, q, velocity) -> None:
+        """the cells running BouzidiVelocityPostProcessor, their
+        BOUZIDI_DISTANCE and BOUZIDI_VELOCITY (n x 19 each); applied after the
+        set_bouzidi list; an empty list removes them"""
+        cells = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=self._dtype).reshape(-1)
+        v = np.ascontiguousarray(velocity, dtype=self._dtype).reshape(-1)
+        assert q.size == Q * cells.size and v.size == Q * cells.size
+        self._f("set_bouzidi_velocity")(self._h, int(cells.size), _ptr(cells), _ptr(q), _ptr(v))
 
     def ini_equilibrium(self, rho, u, sel=None) -> None:
         rho = np.ascontiguousarray(rho, dtype=self._dtype).reshape(-1)

@@ -29,6 +29,8 @@
  *   olb_ref poiseuille_interpolated N OUT [STEPS]  its forced interpolated row (InterpolatedVelocity
  *                                                wall); _nonforced: InterpolatedVelocity wall and
  *                                                inflow, InterpolatedPressure outflow
+ *   olb_ref poiseuille_bouzidi_nonforced N OUT [STEPS]  its non-forced bouzidi rows (Bouzidi wall,
+ *                                                velocity Bouzidi inflow, InterpolatedPressure outflow)
  *   olb_ref normals IN OUT                       discrete normals (boundary::set's classifier)
  *                                                of materials 2 and 3 on the geometry in IN
  *   olb_ref lv_geometry IN STEPS ULID RE OUT     LocalVelocity walls (2) and lid (3) on the
@@ -567,7 +569,7 @@ int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir
 // computeRhoU after initialize (the velocity and density the boundary momenta
 // hold) and the analytic velocity and pressure of the fluid.
 int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSteps = -1,
-                         bool interpolated = false)
+                         bool interpolated = false, bool bouzidi = false)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
   const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
@@ -615,13 +617,23 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
     L.template defineDynamics<BGKdynamics>(G, 1);
     L.template defineDynamics<BGKdynamics>(G, 2);
   }
-  // the interpolated row (testPoiseuille3DBGK.cpp:158-160, 183-197)
-  if (interpolated) {
+  // the interpolated row (testPoiseuille3DBGK.cpp:158-160, 183-197) and the
+  // non-forced bouzidi rows (lines 141-151, 168-172, 195-197, 239-241)
+  Vector<T,3> b0(-dx * 0.2 - 0.5 * dx, r, r), b1(lx + 0.5 * dx, r, r);
+  IndicatorCylinder3D<T> bouzidiPipe(b0, b1, r);
+  if (bouzidi) {
+    L.template defineDynamics<NoDynamics>(G, 2);
+    setBouzidiBoundary<T, DESCRIPTOR, BouzidiPostProcessor>(L, G, 2, bouzidiPipe);
+    L.template defineDynamics<NoDynamics>(G, 3);
+    setBouzidiBoundary<T, DESCRIPTOR, BouzidiVelocityPostProcessor>(L, G, 3, bouzidiPipe);
+    L.template defineDynamics<BGKdynamics>(G, 4);
+    boundary::set<boundary::InterpolatedPressure>(L, G, 4);
+  } else if (interpolated) {
     boundary::set<boundary::InterpolatedVelocity>(L, G, 2);
   } else {
     boundary::set<boundary::LocalVelocity>(L, G, 2);
   }
-  if (!forced) {
+  if (!forced && !bouzidi) {
     L.template defineDynamics<BGKdynamics>(G, 3);
     L.template defineDynamics<BGKdynamics>(G, 4);
     if (interpolated) {
@@ -661,6 +673,9 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
     for (int m = 1; m <= 4; ++m) {
       L.defineRhoU(G, m, rho, poiseuilleU);
       L.iniEquilibrium(G, m, rho, poiseuilleU);
+      if (m == 3 && bouzidi) {
+        setBouzidiVelocity(L, G, 3, poiseuilleU);
+      }
     }
   }
   const T omega = converter.getLatticeRelaxationFrequency();
@@ -737,7 +752,8 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
                 "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu",
-                interpolated ? "interpolated" : "local", forced ? 1 : 0, n, iT, (double)omega, (double)force0,
+                bouzidi ? "bouzidi" : (interpolated ? "interpolated" : "local"), forced ? 1 : 0, n, iT,
+                (double)omega, (double)force0,
                 (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
                 converter.getLatticeTime(maxPhysT));
@@ -752,6 +768,23 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   d.array("energy", "f8", energy);
   d.array("usol", tname(), usol);
   d.array("psol", tname(), psol);
+  if (bouzidi) {
+    // BOUZIDI_DISTANCE (-1: none) and BOUZIDI_VELOCITY of every cell,
+    // population-major
+    auto& b = L.getBlock(0);
+    const std::size_t nc = b.getNcells();
+    std::vector<T> q(DESCRIPTOR::q * nc), v(DESCRIPTOR::q * nc);
+    for (std::size_t c = 0; c < nc; ++c) {
+      const auto qc = b.get(c).template getField<descriptors::BOUZIDI_DISTANCE>();
+      const auto vc = b.get(c).template getField<descriptors::BOUZIDI_VELOCITY>();
+      for (int i = 0; i < DESCRIPTOR::q; ++i) {
+        q[i * nc + c] = qc[i];
+        v[i * nc + c] = vc[i];
+      }
+    }
+    d.array("bouzidi", tname(), q);
+    d.array("bouzidi_velocity", tname(), v);
+  }
   return 0;
 }
 
@@ -926,6 +959,10 @@ int main(int argc, char** argv)
     if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && (argc == 4 || argc == 5)) {
       return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local",
                                   argc == 5 ? std::stoi(argv[4]) : -1);
+    }
+    if (mode == "poiseuille_bouzidi_nonforced" && (argc == 4 || argc == 5)) {
+      return run_poiseuille_local(std::stoi(argv[2]), argv[3], false, argc == 5 ? std::stoi(argv[4]) : -1, false,
+                                  true);
     }
     if ((mode == "poiseuille_interpolated" || mode == "poiseuille_interpolated_nonforced") &&
         (argc == 4 || argc == 5)) {

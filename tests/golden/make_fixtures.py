@@ -237,6 +237,8 @@ def main() -> None:
                 poiseuille_local()
             elif name == "poiseuille_interpolated":
                 poiseuille_interpolated()
+            elif name == "poiseuille_bouzidi_nonforced":
+                poiseuille_bouzidi_nonforced()
             elif name == "normals":
                 normals_fixture()
             elif name == "step_cavity":
@@ -519,6 +521,70 @@ def step_cavity_fixture(res: int = 10, steps: int = 100) -> None:
     }
     np.savez_compressed(os.path.join(HERE, f"step_cavity_res{res}_f64.npz"), **out)
     print("wrote step_cavity", meta)
+
+
+
+def poiseuille_bouzidi_nonforced(n: int = 11) -> None:
+    """testPoiseuille3DBGK's non-forced bouzidi row at N = 11 (FP64, one
+    thread, run to convergence): BouzidiPostProcessor on the wall,
+    BouzidiVelocityPostProcessor on the inflow disc (setBouzidiVelocity with
+    the Poiseuille profile), InterpolatedPressure on the outflow disc.
+    Besides the arrays of the other rows it holds the two processor lists:
+    a cell runs BouzidiPostProcessor when one of its distances points at a
+    wall cell (material 2) and BouzidiVelocityPostProcessor when one points
+    at an inflow cell (material 3); each applies all of the cell's
+    distances, plain before velocity (the reference's operator-type order)"""
+    from openlb_amd import bouzidi as bz
+
+    name = f"poiseuille{n}_nonforced_bouzidi_f64"
+    path = os.path.join(RUN, "poiseuille_bouzidi_nonforced.bin")
+    run("olb_ref_d", ["poiseuille_bouzidi_nonforced", str(n), path], threads=1)
+    meta, arrays = read_dump(path)
+    mats = arrays["materials"]
+    p = meta["padding"]
+    shape = (meta["nx"] + 2 * p, meta["ny"] + 2 * p, meta["nz"] + 2 * p)
+    count = int(np.prod(shape))
+    q = np.asarray(arrays["bouzidi"]).reshape(19, count)
+    v = np.asarray(arrays["bouzidi_velocity"]).reshape(19, count)
+    step = np.array([shape[1] * shape[2], shape[2], 1], dtype=np.int64)
+    d = bz.C @ step
+    lists = {}
+    for solid, key in ((2, "bz"), (3, "bzv")):
+        owner = np.zeros(count, dtype=bool)
+        for j in range(1, 19):
+            has = q[j] >= 0
+            cells = np.flatnonzero(has)
+            target = cells + d[j]
+            ok = (target >= 0) & (target < count)
+            owner[cells[ok][mats[target[ok]] == solid]] = True
+        field = np.where(owner[None, :], q, -1.0)
+        cells, qq = bz.entries(field, p, shape)
+        lists[key + "_cells"] = cells.astype(np.int64)
+        lists[key + "_q"] = qq
+        if key == "bzv":
+            lists["bzv_v"] = v[:, cells].T.copy()
+    keep = np.flatnonzero(mats != 0)
+    fluid = np.flatnonzero(mats == 1)
+    nq = arrays["f0"].size // 19
+    out = {
+        "meta": np.array(json.dumps(meta)),
+        "materials": mats.astype(np.uint8),
+        "distances": arrays["distances"],
+        "cells": keep.astype(np.int64),
+        "f0": arrays["f0"].reshape(19, nq)[:, keep],
+        "fluid": fluid.astype(np.int64),
+        "f1_fluid": arrays["f1"].reshape(19, nq)[:, fluid],
+        "energy": arrays["energy"],
+        "usol_fluid": arrays["usol"].reshape(nq, 3)[fluid],
+        "psol_fluid": arrays["psol"][fluid],
+        "rho0": arrays["rho0"][keep],
+        "u0": arrays["u0"].reshape(nq, 3)[keep],
+        "normal_type": np.asarray(arrays["normal_type"]).astype(np.int8),
+        "normals": np.asarray(arrays["normals"]).astype(np.int8),
+        **lists,
+    }
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, meta, {k: v.shape for k, v in lists.items()})
 
 
 if __name__ == "__main__":

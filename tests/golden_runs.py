@@ -73,6 +73,15 @@ def dynamics_for(meta: dict, storage: str) -> dict:
             return {1: bulk, 2: vel(omega=meta["omega"], base=bulk),
                     3: vel(omega=meta["omega"], base=bulk),
                     4: pre(omega=meta["omega"], base=bulk)}
+        if meta.get("boundary") == "bouzidi" and not meta.get("forced", 1):
+            # testPoiseuille3DBGK.cpp:141-151, 168-197: NoDynamics wall and
+            # inflow (their Bouzidi post-processors come with the fixture's
+            # lists), InterpolatedPressure on the outflow
+            from openlb_amd.cases import InterpolatedPressure
+
+            bulk = Dynamics(BGK, omega=meta["omega"])
+            return {1: bulk, 2: Dynamics(NO_DYNAMICS), 3: Dynamics(NO_DYNAMICS),
+                    4: InterpolatedPressure(omega=meta["omega"], base=bulk)}
         wall = Dynamics(NO_DYNAMICS) if meta.get("boundary") == "bouzidi" else Dynamics(BOUNCE_BACK)
         return {1: forced, 2: wall}
     raise ValueError(case)

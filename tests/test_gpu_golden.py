@@ -411,3 +411,25 @@ def test_gpu_poiseuille_nonforced_interpolated_storage(storage):
     err = rel_l2(ug, ur)
     print(storage, "nonforced interpolated velocity rel L2 vs reference FP64", err)
     assert err <= (1e-5 if storage == "f32" else 2e-3)
+
+
+def test_gpu_poiseuille_nonforced_bouzidi_matches_reference():
+    """the non-forced bouzidi pipe (Bouzidi wall, velocity Bouzidi inflow,
+    InterpolatedPressure outflow, N = 11) on the GPU in FP64 against the
+    reference's run: iT = 490, energies within 1e-10, velocity and pressure
+    L2 errors within 1e-9, fluid populations within 1e-12"""
+    from tests.test_golden import _pressure_error_l2
+
+    meta, d = load("poiseuille11_nonforced_bouzidi_f64")
+    lat = build(Lattice, meta, d, "f64")
+    lat.set_bouzidi(d["bz_cells"], d["bz_q"])
+    lat.set_bouzidi_velocity(d["bzv_cells"], d["bzv_q"], d["bzv_v"])
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 490
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    perr = _pressure_error_l2(lat, meta, d)
+    print("nonforced bouzidi gpu velocity", err, "pressure", perr)
+    assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
