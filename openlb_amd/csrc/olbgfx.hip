@@ -135,15 +135,11 @@ struct Storage<bf16_t> {
     return __uint_as_float((uint32_t)b.bits << 16);
   }
   static __device__ __forceinline__ float load(const bf16_t* p) { return unpack(*p); }
-  // round to nearest even; NaN stays a quiet NaN
+  // round to nearest even: gfx950's v_cvt_pk_bf16_f32, one instruction in
+  // place of the integer rounding sequence
   static __device__ __forceinline__ bf16_t pack(float v)
   {
-    uint32_t x = __float_as_uint(v);
-    if ((x & 0x7fffffffu) > 0x7f800000u) {
-      return bf16_t{(uint16_t)((x >> 16) | 0x40u)};
-    }
-    x += 0x7fffu + ((x >> 16) & 1u);
-    return bf16_t{(uint16_t)(x >> 16)};
+    return bf16_t{__builtin_bit_cast(uint16_t, (__bf16)v)};
   }
 };
 
