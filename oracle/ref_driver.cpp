@@ -31,6 +31,8 @@
  *                                                inflow, InterpolatedPressure outflow
  *   olb_ref poiseuille_bouzidi_nonforced N OUT [STEPS]  its non-forced bouzidi rows (Bouzidi wall,
  *                                                velocity Bouzidi inflow, InterpolatedPressure outflow)
+ *   olb_ref poiseuille_bounceback_nonforced N OUT [STEPS]  its non-forced bounceBack row (BounceBack
+ *                                                wall, interpolated inflow and outflow)
  *   olb_ref normals IN OUT                       discrete normals (boundary::set's classifier)
  *                                                of materials 2 and 3 on the geometry in IN
  *   olb_ref lv_geometry IN STEPS ULID RE OUT     LocalVelocity walls (2) and lid (3) on the
@@ -569,7 +571,7 @@ int run_vtk_tgv(int n, int steps, const std::string& out, const std::string& dir
 // computeRhoU after initialize (the velocity and density the boundary momenta
 // hold) and the analytic velocity and pressure of the fluid.
 int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSteps = -1,
-                         bool interpolated = false, bool bouzidi = false)
+                         bool interpolated = false, bool bouzidi = false, bool bounceBack = false)
 {
   using DESCRIPTOR = descriptors::D3Q19<descriptors::FORCE>;
   const T lx = 2, diameter = 1, r = 0.5, Re = 10, maxPhysT = 20, physInterval = 0.25, residuum = 1e-5;
@@ -628,6 +630,10 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
     setBouzidiBoundary<T, DESCRIPTOR, BouzidiVelocityPostProcessor>(L, G, 3, bouzidiPipe);
     L.template defineDynamics<BGKdynamics>(G, 4);
     boundary::set<boundary::InterpolatedPressure>(L, G, 4);
+  } else if (bounceBack) {
+    // the non-forced bounceBack row: BounceBack wall, interpolated inflow
+    // and outflow (lines 137-139, 172-197)
+    L.template defineDynamics<BounceBack>(G, 2);
   } else if (interpolated) {
     boundary::set<boundary::InterpolatedVelocity>(L, G, 2);
   } else {
@@ -636,7 +642,7 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   if (!forced && !bouzidi) {
     L.template defineDynamics<BGKdynamics>(G, 3);
     L.template defineDynamics<BGKdynamics>(G, 4);
-    if (interpolated) {
+    if (interpolated || bounceBack) {
       boundary::set<boundary::InterpolatedVelocity>(L, G, 3);
       boundary::set<boundary::InterpolatedPressure>(L, G, 4);
     } else {
@@ -752,7 +758,8 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
                 "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu",
-                bouzidi ? "bouzidi" : (interpolated ? "interpolated" : "local"), forced ? 1 : 0, n, iT,
+                bouzidi ? "bouzidi" : (bounceBack ? "bounceBack" : (interpolated ? "interpolated" : "local")),
+                forced ? 1 : 0, n, iT,
                 (double)omega, (double)force0,
                 (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
@@ -959,6 +966,10 @@ int main(int argc, char** argv)
     if ((mode == "poiseuille_local" || mode == "poiseuille_local_nonforced") && (argc == 4 || argc == 5)) {
       return run_poiseuille_local(std::stoi(argv[2]), argv[3], mode == "poiseuille_local",
                                   argc == 5 ? std::stoi(argv[4]) : -1);
+    }
+    if (mode == "poiseuille_bounceback_nonforced" && (argc == 4 || argc == 5)) {
+      return run_poiseuille_local(std::stoi(argv[2]), argv[3], false, argc == 5 ? std::stoi(argv[4]) : -1, false,
+                                  false, true);
     }
     if (mode == "poiseuille_bouzidi_nonforced" && (argc == 4 || argc == 5)) {
       return run_poiseuille_local(std::stoi(argv[2]), argv[3], false, argc == 5 ? std::stoi(argv[4]) : -1, false,

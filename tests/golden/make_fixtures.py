@@ -239,6 +239,11 @@ def main() -> None:
                 poiseuille_interpolated()
             elif name == "poiseuille_bouzidi_nonforced":
                 poiseuille_bouzidi_nonforced()
+            elif name == "poiseuille_bounceback_nonforced":
+                # the non-forced bounceBack row: BounceBack wall, interpolated
+                # inflow and outflow (one thread, as the interpolated rows)
+                poiseuille_rows((("poiseuille_bounceback_nonforced", "poiseuille11_nonforced_bounceback_f64"),),
+                                threads=1)
             elif name == "normals":
                 normals_fixture()
             elif name == "step_cavity":

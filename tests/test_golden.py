@@ -327,3 +327,24 @@ def test_oracle_poiseuille_nonforced_bouzidi_matches_reference():
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_oracle_poiseuille_nonforced_bounceback_known_answer():
+    """testPoiseuille3DBGK.cpp:61 (non-forced, bounceBack, N = 11):
+    BounceBack wall, InterpolatedVelocity inflow and InterpolatedPressure
+    outflow; converges at the test's iT = 621 with velocity L2 error
+    0.1284974 and pressure L2 error 0.40862686 (+- 1e-6), within 1e-9 of the
+    checker's run; energies within 1e-10, fluid populations within 1e-12"""
+    meta, d = load("poiseuille11_nonforced_bounceback_f64")
+    lat = build(OracleLattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    print("iT", it, "checker", meta["iT"])
+    assert it == meta["iT"] == 621
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    perr = _pressure_error_l2(lat, meta, d)
+    print("velocity", err, "pressure", perr)
+    assert abs(err - 0.1284974) <= 1e-6 and abs(err - meta["velocityErrorL2"]) <= 1e-9
+    assert abs(perr - 0.40862686) <= 1e-6 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12

@@ -433,3 +433,24 @@ def test_gpu_poiseuille_nonforced_bouzidi_matches_reference():
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_gpu_poiseuille_nonforced_bounceback_known_answer():
+    """testPoiseuille3DBGK.cpp:61 (non-forced, bounceBack, N = 11) on the GPU
+    in FP64: BounceBack wall, InterpolatedVelocity inflow, InterpolatedPressure
+    outflow; iT = 621, velocity and pressure L2 errors within 1e-6 of the
+    test's 0.1284974 and 0.40862686, fluid populations within 1e-12 of the
+    reference"""
+    from tests.test_golden import _pressure_error_l2
+
+    meta, d = load("poiseuille11_nonforced_bounceback_f64")
+    lat = build(Lattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == 621
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    perr = _pressure_error_l2(lat, meta, d)
+    print("nonforced bounceback gpu velocity", err, "pressure", perr)
+    assert abs(err - 0.1284974) <= 1e-6 and abs(perr - 0.40862686) <= 1e-6
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
