@@ -523,6 +523,22 @@ public:
     _dirty = true;
   }
 
+  /// the cells that run BouzidiVelocityPostProcessor with their 19
+  /// distances and 19 BOUZIDI_VELOCITY coefficients (c_i . u_wall at the wall
+  /// intersection, as setBouzidiVelocity writes them,
+  /// boundary/setBouzidiBoundary.h:340-420); applied after the setBouzidi
+  /// list; replaces any earlier velocity list
+  void setBouzidiVelocity(std::vector<std::int64_t> cells, std::vector<double> q, std::vector<double> velocity)
+  {
+    if (q.size() != (std::size_t)DESCRIPTOR::q * cells.size() || velocity.size() != q.size()) {
+      throw std::invalid_argument("19 Bouzidi distances and 19 velocity coefficients per cell");
+    }
+    _bouzidiVelocityCells = std::move(cells);
+    _bouzidiVelocityQ = std::move(q);
+    _bouzidiVelocityV = std::move(velocity);
+    _dirty = true;
+  }
+
   /// LocalPressure on the flat faces of `material` (boundary::set<boundary::
   /// LocalPressure>, boundary/localPressure3D.h:47-58): regularised faces
   /// with the density the RHO field (defineRho) or 1 gives and the velocity's
@@ -663,6 +679,9 @@ public:
     check(olbgfx_set_cell_normals(handle(), _normals.empty() ? nullptr : _normals.data()));
     check(olbgfx_set_bouzidi(handle(), (int64_t)_bouzidiCells.size(), _bouzidiCells.data(), OLBGFX_VALUE_F64,
                              _bouzidiQ.data()));
+    check(olbgfx_set_bouzidi_velocity(handle(), (int64_t)_bouzidiVelocityCells.size(),
+                                      _bouzidiVelocityCells.data(), OLBGFX_VALUE_F64, _bouzidiVelocityQ.data(),
+                                      _bouzidiVelocityV.data()));
     for (auto& kv : _fields) {
       check(olbgfx_set_cell_field(handle(), kv.first, valueType(),
                                   kv.second.uniform ? nullptr : kv.second.values.data()));
@@ -960,6 +979,9 @@ private:
   std::vector<std::int8_t> _normals;  // discrete normal per cell for LocalVelocity
   std::vector<std::int64_t> _bouzidiCells;
   std::vector<double> _bouzidiQ;
+  std::vector<std::int64_t> _bouzidiVelocityCells;
+  std::vector<double> _bouzidiVelocityQ;
+  std::vector<double> _bouzidiVelocityV;
   std::map<int32_t, Field> _fields;
   std::vector<T> _host;
   bool _hostValid = false;

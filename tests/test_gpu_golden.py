@@ -454,3 +454,52 @@ def test_gpu_poiseuille_nonforced_bounceback_known_answer():
     assert abs(err - 0.1284974) <= 1e-6 and abs(perr - 0.40862686) <= 1e-6
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_gpu_bouzidi_velocity_list_checks_and_wall_term():
+    """olbgfx_set_bouzidi_velocity takes the rules of olbgfx_set_bouzidi
+    (core cells, ascending, finite values, no cross reads) and a finite
+    coefficient per distance; with all coefficients zero it gives the plain
+    Bouzidi result, and a coefficient w on a q <= 1/2 distance lowers the
+    written population by exactly 2 w t_i / cs^2 (FP64)"""
+    from openlb_amd import OlbgfxError
+    from tests.helpers import T as W
+
+    L = Lattice((6, 6, 6), padding=1, storage="f64")
+    a, b = L.cell_id(2, 2, 2), L.cell_id(3, 2, 2)
+    q = np.full((1, 19), -1.0)
+    q[0, 10] = 0.3
+    zero = np.zeros((1, 19))
+    with pytest.raises(OlbgfxError):  # a padding cell
+        L.set_bouzidi_velocity([0], q, zero)
+    bad = zero.copy()
+    bad[0, 10] = np.inf
+    with pytest.raises(OlbgfxError):
+        L.set_bouzidi_velocity([a], q, bad)
+    qb = np.full((1, 19), -1.0)
+    qb[0, 1] = 0.3
+    with pytest.raises(OlbgfxError):  # a and b read each other's updates
+        L.set_bouzidi_velocity([a, b], np.vstack([q, qb]), np.vstack([zero, zero]))
+    rng = np.random.default_rng(3)
+    f0 = 0.01 * rng.standard_normal((19, L.ncells))
+    out = []
+    for mode, w in (("plain", 0.0), ("velocity", 0.0), ("velocity", 0.02)):
+        L.set_bouzidi([], np.zeros((0, 19)))
+        L.set_bouzidi_velocity([], np.zeros((0, 19)), np.zeros((0, 19)))
+        L.set_populations(f0)
+        if mode == "plain":
+            L.set_bouzidi([a], q)
+        else:
+            v = zero.copy()
+            v[0, 10] = w
+            L.set_bouzidi_velocity([a], q, v)
+        L.post_process()
+        out.append(L.get_populations())
+    np.testing.assert_array_equal(out[1], out[0])
+    o = 1  # opposite of direction 10
+    diff = out[1][o, a] - out[2][o, a]
+    assert abs(diff - 2 * 0.02 * W[10] * 3) <= 1e-15
+    rest = np.ones_like(out[1], dtype=bool)
+    rest[o, a] = False
+    np.testing.assert_array_equal(out[2][rest], out[1][rest])
+    L.close()
