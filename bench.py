@@ -380,8 +380,9 @@ def main():
         os.environ["OLBGFX_TUNE_CACHE"] = os.path.abspath(args.tune_cache)
         USE_TUNE_CACHE = True
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (a lattice shards into x-slabs "
+              "whose halo planes olbgfx_halo_pack/unpack build; the transport between GPUs is not built)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -1,10 +1,10 @@
-"""Build recipes for the gfx950 platform library and the CPU checker.
+"""Build recipes for the gfx950 platform library and the CPU oracle.
 
 ``build_library`` compiles ``openlb_amd/csrc/olbgfx.hip`` with hipcc for
 gfx950 into ``openlb_amd/libolbgfx.so`` (in-tree, so it travels with the
 source tree).  ``build_oracle`` compiles the C restatement in ``oracle/`` and,
 when the OpenLB sources are present, the reference driver in ``oracle/_ref``;
-both are checkers used only by the tests and the CPU baseline.
+both are test oracles used only by the tests and the CPU baseline.
 """
 from __future__ import annotations
 
@@ -66,7 +66,7 @@ def build_host_examples() -> None:
 
 
 def build_oracle(with_reference: bool = True) -> None:
-    """Compile the CPU restatement (and the reference checker if possible)."""
+    """Compile the CPU restatement (and the reference build if possible)."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     if with_reference and os.path.isdir("/root/reference/src"):
         _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])

@@ -1,7 +1,7 @@
 /*
  * olb_oracle.c -- CPU restatement of OpenLB's D3Q19 collide-and-stream step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
+ * TEST INFRASTRUCTURE ONLY.  This file is a test oracle: it is loaded by the
  * tests under tests/, by __graft_entry__.smoke() and by the cpu_baseline leg
  * of bench.py, and nowhere else.  The product library (libolbgfx) never links,
  * loads or calls it.

@@ -1,7 +1,7 @@
 """ctypes access to the CPU restatement in olb_oracle.c.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path
+cpu_baseline leg of bench.py, as the oracle.  The product path
 (openlb_amd, libolbgfx.so) never imports it.
 
 ``OracleLattice`` has the same methods as ``openlb_amd.Lattice`` so a test

@@ -4,7 +4,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile against the OpenLB
  * sources where they lie (never copied); the binary lands in oracle/_ref/ and
- * is a checker: tests compare the gfx950 platform and the C restatement with
+ * is a test oracle: tests compare the gfx950 platform and the C restatement with
  * what it writes.  It is our own driver code calling the public OpenLB API
  * (SuperGeometry, SuperLattice, dynamics/boundary setters), the way an OpenLB
  * case does.

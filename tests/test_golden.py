@@ -203,11 +203,11 @@ def test_oracle_poiseuille_nonforced_local_known_answer():
 def test_oracle_poiseuille_forced_local_known_answer():
     """testPoiseuille3DBGK.cpp:56 (forced, local, N = 11; periodic along the
     pipe): converges at iT = 317 with velocity L2 error 0.020342 +- 1e-6, the
-    test's own numbers, and within 3e-6 of the checker's run (0.0203446).
+    test's own numbers, and within 3e-6 of the reference's run (0.0203446).
     The internal edges' density reads populations entering across the other
     wall from cells outside the pipe, whose values come from the padding at
     the periodic faces and are defined by no implementation; from identical
-    states the oracle's step equals the checker's, and the runs agree to
+    states the oracle's step equals the reference's, and the runs agree to
     1e-16 until the fifth step, where those values first differ."""
     meta, d = load("poiseuille11_forced_local_f64")
     lat = build(OracleLattice, meta, d, "f64")
@@ -215,7 +215,7 @@ def test_oracle_poiseuille_forced_local_known_answer():
     assert it == 317 == meta["iT"]
     np.testing.assert_allclose(energy[:4], d["energy"][:4], rtol=1e-12)
     err = velocity_error_l2(lat, meta, d)
-    print("velocity", err, "checker", meta["velocityErrorL2"])
+    print("velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - 0.020342) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 3e-6
 
@@ -266,7 +266,7 @@ def test_oracle_poiseuille_nonforced_interpolated_known_answer():
     PlaneFdBoundaryProcessor3D, LocalVelocity's edges and corners),
     InterpolatedPressure on the outflow disc; converges at iT = 570 with
     velocity L2 error 0.04976679 and pressure L2 error 0.10483063 (+- 1e-6
-    each) and within 1e-9 of the checker's run (one thread: the reference
+    each) and within 1e-9 of the reference's run (one thread: the reference
     applies the face processors under omp parallel for, and an outflow face
     reads its neighbours' velocity from populations another thread may be
     rewriting); energies within 1e-10, fluid populations within 1e-12"""
@@ -287,7 +287,7 @@ def test_oracle_poiseuille_nonforced_interpolated_known_answer():
 def test_oracle_poiseuille_forced_interpolated_known_answer():
     """testPoiseuille3DBGK.cpp:57 (forced, interpolated, N = 11; periodic
     along the pipe): converges at iT = 303 with velocity L2 error 0.016909
-    +- 1e-6, the test's own numbers.  The checker built here stops at
+    +- 1e-6, the test's own numbers.  The reference built here stops at
     iT = 302 with 0.0168319: the face processors at the periodic ends read
     the velocity of wall cells in the overlap (too close to the block edge
     for boundary::set, so plain ForcedBGK), computed from populations that
@@ -300,7 +300,7 @@ def test_oracle_poiseuille_forced_interpolated_known_answer():
     assert it == 303
     np.testing.assert_allclose(energy[:5], d["energy"][:5], rtol=1e-12)
     err = velocity_error_l2(lat, meta, d)
-    print("velocity", err, "checker", meta["velocityErrorL2"])
+    print("velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - 0.016909) <= 1e-6
 
 
@@ -334,11 +334,11 @@ def test_oracle_poiseuille_nonforced_bounceback_known_answer():
     BounceBack wall, InterpolatedVelocity inflow and InterpolatedPressure
     outflow; converges at the test's iT = 621 with velocity L2 error
     0.1284974 and pressure L2 error 0.40862686 (+- 1e-6), within 1e-9 of the
-    checker's run; energies within 1e-10, fluid populations within 1e-12"""
+    reference's run; energies within 1e-10, fluid populations within 1e-12"""
     meta, d = load("poiseuille11_nonforced_bounceback_f64")
     lat = build(OracleLattice, meta, d, "f64")
     it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
-    print("iT", it, "checker", meta["iT"])
+    print("iT", it, "reference", meta["iT"])
     assert it == meta["iT"] == 621
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)

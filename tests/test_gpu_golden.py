@@ -291,7 +291,7 @@ def test_gpu_poiseuille_nonforced_local_known_answer():
 def test_gpu_poiseuille_forced_local_known_answer():
     """testPoiseuille3DBGK.cpp:56 (forced, local, N = 11, periodic) on the GPU
     in FP64: iT = 317; velocity L2 error within 1e-6 of the reference run by
-    the checker (0.0203446) and within 3e-6 of the test's 0.020342.  The
+    the reference run (0.0203446) and within 3e-6 of the test's 0.020342.  The
     internal edges' density reads populations that enter across the other
     wall from cells outside the pipe, whose values the reference never
     defines (they come from the padding at the periodic faces), so the last
@@ -301,7 +301,7 @@ def test_gpu_poiseuille_forced_local_known_answer():
     it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
     assert it == 317
     err = velocity_error_l2(lat, meta, d)
-    print("forced local gpu velocity", err, "checker", meta["velocityErrorL2"])
+    print("forced local gpu velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - meta["velocityErrorL2"]) <= 1e-6
     assert abs(err - 0.020342) <= 3e-6
 
@@ -378,18 +378,18 @@ def test_gpu_poiseuille_forced_interpolated_known_answer():
     on the GPU in FP64.  As in the forced local row, the face and edge
     cells at the periodic ends read populations that entered from the
     overlap's corners (outside the pipe in y or z and beyond the periodic
-    face in x), which no implementation defines: the checker stops at
+    face in x), which no implementation defines: the reference stops at
     iT = 302 with 0.0168319, the oracle (which fills the whole overlap from
     the periodic images) at the test's iT = 303 with 0.016908.  The GPU
     writes only the populations that cross a periodic face into the
     overlap and refreshes the overlap cells the processors read; it stops
-    at iT = 302 like the checker, with the error within 1e-5 of the
-    checker's and 1e-4 of the test's 0.016909"""
+    at iT = 302 like the reference, with the error within 1e-5 of the
+    reference's and 1e-4 of the test's 0.016909"""
     meta, d = load("poiseuille11_forced_interpolated_f64")
     lat = build(Lattice, meta, d, "f64")
     it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
     err = velocity_error_l2(lat, meta, d)
-    print("forced interpolated gpu", it, "velocity", err, "checker", meta["iT"], meta["velocityErrorL2"])
+    print("forced interpolated gpu", it, "velocity", err, "reference", meta["iT"], meta["velocityErrorL2"])
     assert it == meta["iT"] == 302
     assert abs(err - meta["velocityErrorL2"]) <= 1e-5
     assert abs(err - 0.016909) <= 1e-4
