@@ -621,7 +621,12 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   }
   // the interpolated row (testPoiseuille3DBGK.cpp:158-160, 183-197) and the
   // non-forced bouzidi rows (lines 141-151, 168-172, 195-197, 239-241)
-  Vector<T,3> b0(-dx * 0.2 - 0.5 * dx, r, r), b1(lx + 0.5 * dx, r, r);
+  // the Bouzidi pipe of prepareLattice (testPoiseuille3DBGK.cpp:130-131,
+  // 145-146): a fresh cylinder from x = -dx/2 to lx + dx/2, not the
+  // geometry's pipe (which starts at -0.2 dx)
+  Vector<T,3> b0(0, r, r), b1(lx, r, r);
+  b0[0] -= 0.5 * dx;
+  b1[0] += 0.5 * dx;
   IndicatorCylinder3D<T> bouzidiPipe(b0, b1, r);
   if (bouzidi) {
     L.template defineDynamics<NoDynamics>(G, 2);

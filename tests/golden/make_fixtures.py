@@ -127,7 +127,8 @@ def checkpoint_fixtures() -> None:
     print("wrote checkpoint fixtures", meta)
 
 
-def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode: str = "cavity") -> None:
+def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode: str = "cavity",
+                   ulid: float = 0.1, re: float = 100.0, name: str | None = None, threads: int = 4) -> None:
     """the cavity3dBenchmark geometry at RESOLUTION 64, Re = 100 (tau = 0.692),
     lid u = 0.1, 1000 steps from rest; mode "cavity": BounceBack walls and a
     BounceBackVelocity lid, mode "cavity_lv": LocalVelocity walls and lid.
@@ -135,7 +136,7 @@ def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode
     velocity on the mid-plane z = res/2, the populations of 256 seeded fluid
     cells, and FP64 sums over the fluid."""
     path = os.path.join(RUN, f"{mode}{res}_{tag}.bin")
-    run(binary, [mode, str(res), str(steps), "0.1", "100", path])
+    run(binary, [mode, str(res), str(steps), repr(float(ulid)), repr(float(re)), path], threads=threads)
     meta, arrays = read_dump(path)
     q = 19
     n = arrays["f1"].size // q
@@ -155,13 +156,40 @@ def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode
     sample = np.sort(rng.choice(fl, 256, replace=False))
     uf = u.reshape(-1, 3)[fluid.reshape(-1)]
     sums = np.array([(rho.reshape(-1)[fluid.reshape(-1)] - 1).sum(), *uf.sum(0), (uf ** 2).sum()])
-    name = f"cavity3d{'_lv' if mode == 'cavity_lv' else ''}_res{res}_{steps}steps_{tag}"
+    if name is None:
+        name = f"cavity3d{'_lv' if mode == 'cavity_lv' else ''}_res{res}_{steps}steps_{tag}"
     np.savez_compressed(os.path.join(HERE, name + ".npz"),
                         meta=np.array(json.dumps(meta)), materials=arrays["materials"].astype(np.uint8),
                         u_midplane=mid, sample_cells=sample.astype(np.int64),
                         f1_sample=f[:, sample].astype(np.float32 if tag == "f32" else np.float64),
                         fluid_sums=sums)
     print("wrote", name, sums)
+
+
+ENSEMBLE_STEPS = 1000
+
+
+def ensemble_member_picks(members) -> dict:
+    """the smallest, the median and the largest member of the 64-cavity
+    ensemble by edge length (first index on ties)"""
+    edges = [m.extent[0] for m in members]
+    order = sorted(range(len(edges)), key=lambda k: (edges[k], k))
+    return {"smallest": order[0], "median": order[len(order) // 2], "largest": order[-1]}
+
+
+def ensemble_members() -> None:
+    """three members of the 64-cavity ensemble (cases.ensemble_cavities, seed
+    20261202) through the reference's FP32 path, 1000 steps from rest: a
+    member of edge n is the cavity3dBenchmark geometry at RESOLUTION n - 1
+    (n^3 core cells) with its own lid speed U; the reference forms
+    nu = U RES / Re, so Re' = Re (n - 1) / n gives the member's nu = U n / Re"""
+    members = cases.ensemble_cavities(dtype=np.float32)
+    for label, k in ensemble_member_picks(members).items():
+        m = members[k]
+        n = m.extent[0]
+        u = m.info["u_lid"]
+        cavity_reduced("olb_ref_f", "f32", res=n - 1, steps=ENSEMBLE_STEPS, ulid=u, re=100.0 * (n - 1) / n,
+                       name=f"ensemble_member_{label}_{ENSEMBLE_STEPS}steps_f32", threads=8)
 
 
 # full-size workloads, kept as reductions (tests/golden_runs.reduce_state)
@@ -248,6 +276,8 @@ def main() -> None:
                 normals_fixture()
             elif name == "step_cavity":
                 step_cavity_fixture()
+            elif name == "ensemble_members":
+                ensemble_members()
             else:
                 full_size(name)
         return

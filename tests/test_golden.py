@@ -305,26 +305,28 @@ def test_oracle_poiseuille_forced_interpolated_known_answer():
 
 
 def test_oracle_poiseuille_nonforced_bouzidi_matches_reference():
-    """testPoiseuille3DBGK.cpp:63's case (non-forced, bouzidi, N = 11):
+    """testPoiseuille3DBGK.cpp:63 (non-forced, bouzidi, N = 11):
     BouzidiPostProcessor on the wall, BouzidiVelocityPostProcessor with the
     Poiseuille profile (setBouzidiVelocity) on the inflow disc,
-    InterpolatedPressure on the outflow disc.  The reference built from
-    these sources (one thread) converges at iT = 490 with velocity L2 error
-    0.0560916 and pressure L2 error 0.0717667; the oracle matches that run:
-    same iT, energies within 1e-10, both errors within 1e-9, fluid
-    populations within 1e-12.  (The test's table lists 572 / 0.0565442 /
-    0.0701867 for this row, which these sources do not produce.)"""
+    InterpolatedPressure on the outflow disc, the Bouzidi cylinder from
+    x = -dx/2 to lx + dx/2 (lines 130-131, 145-146).  The reference built
+    from these sources (one thread) reproduces the test's row, iT = 572 with
+    velocity L2 error 0.056544195 and pressure L2 error 0.070186678; the
+    oracle matches that run: same iT, energies within 1e-10, both errors
+    within 1e-9 of the reference's and 1e-6 of the table's, fluid populations
+    within 1e-12."""
     meta, d = load("poiseuille11_nonforced_bouzidi_f64")
     lat = build(OracleLattice, meta, d, "f64")
     lat.set_bouzidi(d["bz_cells"], d["bz_q"])
     lat.set_bouzidi_velocity(d["bzv_cells"], d["bzv_q"], d["bzv_v"])
     it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
-    assert it == meta["iT"] == 490
+    assert it == meta["iT"] == 572
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
     perr = _pressure_error_l2(lat, meta, d)
     print("velocity", err, "pressure", perr)
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
+    assert abs(err - 0.056544195) <= 1e-6 and abs(perr - 0.070186678) <= 1e-6
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
 
@@ -348,3 +350,27 @@ def test_oracle_poiseuille_nonforced_bounceback_known_answer():
     assert abs(perr - 0.40862686) <= 1e-6 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+def test_oracle_ensemble_smallest_member_matches_reference():
+    """the smallest cavity of the 64-member ensemble (cases.ensemble_cavities,
+    its own edge and lid speed, Re = 100) against the reference's FP32 run of
+    the same cavity, 1000 steps: this pins the fixture's mapping (edge n =
+    RESOLUTION n - 1, Re' = Re (n - 1) / n) that the GPU ensemble test uses"""
+    from tests.golden.make_fixtures import ENSEMBLE_STEPS, ensemble_member_picks
+    from tests.golden_runs import cavity_reduced
+
+    members = cases.ensemble_cavities(dtype=np.float32)
+    k = ensemble_member_picks(members)["smallest"]
+    c = members[k]
+    meta, d = load(f"ensemble_member_smallest_{ENSEMBLE_STEPS}steps_f32")
+    assert meta["nx"] == c.extent[0] and meta["steps"] == ENSEMBLE_STEPS
+    np.testing.assert_array_equal(c.materials.reshape(-1), d["materials"])
+    lat = OracleLattice(c.extent, c.padding, c.periodic, "f32")
+    c.apply(lat)
+    lat.collide_and_stream(meta["steps"])
+    mid, fs, _ = cavity_reduced(lat, meta, d)
+    p = c.padding
+    fluid = c.materials[p:-p, p:-p, p + meta["res"] // 2] == 1
+    assert np.max(np.abs(mid - d["u_midplane"])[fluid]) <= 1e-6
+    assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5

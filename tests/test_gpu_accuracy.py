@@ -1,0 +1,93 @@
+"""Accuracy of the storage formats at full size on the GPU, each against an
+FP64 run of the same workload from the same start:
+
+- Taylor-Green 512^3 (tau = 0.56, u0 = 0.04), 1000 steps: the velocity's
+  relative L2 error over the core must stay within 1e-3 for FP16 shifted
+  storage; FP32 within 1e-4; BF16 within 1.5e-2 (its 8 significant bits
+  hold the shifted populations to ~4e-3 relative, measured 9.0e-3 here);
+- the porous channel 384x128x128 (ForcedTRT, Lambda = 3/16, g = 1e-6,
+  BounceBack spheres), 5000 steps: FP16 within 2.5e-2 (its velocities of
+  ~1e-4 give shifted populations of ~1e-5, below FP16's smallest normal
+  number 6.1e-5, where its spacing is 6e-8), FP32 within 1e-4.
+
+The FP64 Taylor-Green run itself decays at the analytic rate 2 nu |k|^2
+within 1.5 %."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TGV_BOUND = {"f32": 1e-4, "f16": 1e-3, "bf16": 1.5e-2}
+POROUS_BOUND = {"f32": 1e-4, "f16": 2.5e-2}
+
+
+def _velocity_run(case, storage, steps):
+    from openlb_amd import Lattice
+
+    lat = Lattice(case.extent, case.padding, case.periodic, storage)
+    case.apply(lat)
+    e0 = lat.reduce_moments()["sumUSqr"]
+    lat.collide_and_stream(steps)
+    m = lat.reduce_moments()
+    _, u = lat.compute_rho_u()
+    lat.close()
+    return u, e0, m
+
+
+def _errors(u, u_ref):
+    d = u - u_ref
+    return float(np.sqrt(np.sum(d * d) / np.sum(u_ref * u_ref))), float(np.max(np.abs(d)))
+
+
+@pytest.mark.timeout(900)
+def test_tgv512_storage_accuracy_vs_fp64():
+    from openlb_amd import cases
+
+    n, steps = 512, 1000
+    print(f"[accuracy] tgv{n} {steps} steps f64", flush=True)
+    case = cases.tgv(n, u0=0.04, tau=0.56, dtype=np.float64)
+    core = case.core_mask().reshape(-1)
+    u, e0, m = _velocity_run(case, "f64", steps)
+    del case
+    u_ref = u[core].astype(np.float64)
+    del u
+    nu = (0.56 - 0.5) / 3.0
+    rate = -np.log(m["sumUSqr"] / e0) / steps
+    analytic = 2 * nu * 3 * (2 * np.pi / n) ** 2
+    print(f"f64 energy decay {rate:.6e} per step, analytic {analytic:.6e}", flush=True)
+    assert abs(rate / analytic - 1) <= 1.5e-2
+    for st in ("f32", "f16", "bf16"):
+        print(f"[accuracy] tgv{n} {steps} steps {st}", flush=True)
+        case = cases.tgv(n, u0=0.04, tau=0.56, dtype=np.float32)
+        u, _, _ = _velocity_run(case, st, steps)
+        del case
+        l2, mx = _errors(u[core].astype(np.float64), u_ref)
+        del u
+        print(f"{st}: velocity rel L2 vs f64 {l2:.3e}, max |du|/u0 {mx / 0.04:.3e}", flush=True)
+        assert l2 <= TGV_BOUND[st]
+
+
+@pytest.mark.timeout(900)
+def test_porous_channel_storage_accuracy_vs_fp64():
+    from openlb_amd import cases
+
+    steps = 5000
+    kw = dict(radius=6.0, porosity=0.7, seed=20261202, force=1e-6, tau=0.8, magic=3.0 / 16.0)
+    print(f"[accuracy] porous 384x128x128 {steps} steps f64", flush=True)
+    case = cases.porous_channel((384, 128, 128), dtype=np.float64, **kw)
+    fluid = (case.materials == 1).reshape(-1)
+    u, _, _ = _velocity_run(case, "f64", steps)
+    u_ref = u[fluid].astype(np.float64)
+    del u
+    mean_ref = float(u_ref[:, 0].mean())
+    assert mean_ref > 0
+    for st in ("f32", "f16"):
+        print(f"[accuracy] porous {steps} steps {st}", flush=True)
+        case = cases.porous_channel((384, 128, 128), dtype=np.float32, **kw)
+        u, _, _ = _velocity_run(case, st, steps)
+        uf = u[fluid].astype(np.float64)
+        del u
+        l2, mx = _errors(uf, u_ref)
+        print(f"{st}: velocity rel L2 vs f64 {l2:.3e}, max |du| {mx:.3e}, mean u_x {uf[:, 0].mean():.6e} "
+              f"(f64 {mean_ref:.6e})", flush=True)
+        assert l2 <= POROUS_BOUND[st]

@@ -197,3 +197,82 @@ def test_timed_modes_advance_like_members_alone(mode):
         lat.collide_and_stream(6)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x.get_populations(), y.get_populations())
+
+
+# the batched-ensemble workload at its real size: 64 lid-driven cavities with
+# edges from U{32..128} and lid speeds from U[0.02, 0.1] (seed 20261202),
+# Re = 100; the launch shape and cache-sized group schedule are the ones the
+# autotuner picks for it on MI355X
+CONFIG5_STEPS = 1000
+CONFIG5_LAUNCH = {
+    "f32": {"launch": (256, 4, 2), "order": 2, "schedule": (160 << 20, 8, False)},
+    "f16": {"launch": (256, 0, 4), "order": -1, "schedule": (160 << 20, 8, False)},
+}
+
+
+def _config5(storage):
+    cs = cases.ensemble_cavities(dtype=np.float32)
+    lats = []
+    for c in cs:
+        lat = Lattice(c.extent, c.padding, c.periodic, storage)
+        c.apply(lat)
+        lats.append(lat)
+    return cs, lats
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_config5_ensemble_full_size(storage):
+    """64 cavities (49.5 M cells) advanced 1000 steps by replaying a captured
+    graph (grouped schedule, 40 steps per replay): every member bit for bit
+    equal to the same member stepped alone; the smallest, median and largest
+    member against the reference's own FP32 run of that cavity
+    (tests/golden/make_fixtures.py ensemble_members): mid-plane velocity over
+    the fluid within 1e-6 (FP32) or 2e-4 (FP16 shifted storage) absolute,
+    FP32 sample populations within 1e-5 relative"""
+    import os
+
+    from tests.golden.make_fixtures import ENSEMBLE_STEPS, ensemble_member_picks
+    from tests.golden_runs import GOLDEN, cavity_reduced, load
+
+    assert ENSEMBLE_STEPS == CONFIG5_STEPS
+    print(f"[config5] storage={storage} building 64 members", flush=True)
+    cs, a = _config5(storage)
+    assert sum(int(np.prod(c.extent)) for c in cs) == 49454483
+    cfg = CONFIG5_LAUNCH[storage]
+    ens = Ensemble(a)
+    ens.set_launch(*cfg["launch"])
+    ens.set_block_order(cfg["order"])
+    ens.set_schedule(*cfg["schedule"])
+    ens.capture(40)
+    ens.launch_graph(CONFIG5_STEPS // 40)
+    print(f"[config5] storage={storage} graph replay done", flush=True)
+    picks = ensemble_member_picks(cs)
+    for k, (c, x) in enumerate(zip(cs, a)):
+        y = Lattice(c.extent, c.padding, c.periodic, storage)
+        c.apply(y)
+        y.collide_and_stream(CONFIG5_STEPS)
+        np.testing.assert_array_equal(x.ring_shifts(), y.ring_shifts())
+        np.testing.assert_array_equal(x.get_populations(), y.get_populations(), err_msg=f"member {k}")
+        y.close()
+    print(f"[config5] storage={storage} 64 members bit-exact", flush=True)
+    for label, k in picks.items():
+        name = f"ensemble_member_{label}_{CONFIG5_STEPS}steps_f32"
+        if not os.path.exists(os.path.join(GOLDEN, name + ".npz")):
+            pytest.fail(f"fixture {name} missing")
+        meta, d = load(name)
+        c = cs[k]
+        assert meta["nx"] == c.extent[0] and meta["res"] == c.extent[0] - 1
+        np.testing.assert_array_equal(d["materials"], c.materials.reshape(-1))
+        mid, fs, sums = cavity_reduced(a[k], meta, d)
+        p = c.padding
+        fluid = c.materials[p:-p, p:-p, p + meta["res"] // 2] == 1
+        du = float(np.max(np.abs(mid - d["u_midplane"])[fluid]))
+        print(f"[config5] {storage} {label} member {k} (n={c.extent[0]}, u_lid={c.info['u_lid']:.4f}): "
+              f"max |du| mid-plane {du:.3e}", flush=True)
+        assert du <= (1e-6 if storage == "f32" else 2e-4)
+        if storage == "f32":
+            assert rel_err_unshifted(fs, d["f1_sample"]) <= 1e-5
+    ens.close()
+    for lat in a:
+        lat.close()
