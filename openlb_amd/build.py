@@ -16,7 +16,7 @@ import sys
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "openlb_amd")
 LIB = os.path.join(PKG, "libolbgfx.so")
-HIP_SOURCES = [os.path.join(PKG, "csrc", "olbgfx.hip")]
+HIP_SOURCES = [os.path.join(PKG, "csrc", "olbgfx.hip"), os.path.join(PKG, "csrc", "olbprobe.hip")]
 
 HIPCC_FLAGS = [
     "-O3",
