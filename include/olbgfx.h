@@ -129,6 +129,18 @@ typedef struct {
 olbgfx_status olbgfx_lattice_create(const olbgfx_lattice_desc* desc, olbgfx_lattice** out);
 olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* lattice);
 
+/* number of HIP devices visible (0 when none); checkPlatform<GPU_HIP>
+ * (core/platform/gpu/hip/communicator.hh:45-81) checks the same */
+olbgfx_status olbgfx_device_count(int32_t* count);
+
+/* bound the steps queued ahead of the host: olbgfx_collide_and_stream then
+ * waits, before returning, until step t - steps has finished (0 = no bound,
+ * the default).  A host timer around a run of steps, as the reference's
+ * examples keep (cavity3d.cpp:200-221), then misses at most that many steps.
+ * No reference counterpart (the reference's GPU platform synchronises per
+ * stage, core/superLattice.hh:800-815). */
+olbgfx_status olbgfx_set_inflight_limit(olbgfx_lattice* lattice, uint32_t steps);
+
 /* number of cells of the padded block (BlockStructureD::getNcells,
  * core/blockStructure.h:137-148) */
 olbgfx_status olbgfx_lattice_ncells(const olbgfx_lattice* lattice, uint64_t* ncells);
@@ -196,7 +208,7 @@ olbgfx_status olbgfx_set_bouzidi(olbgfx_lattice* lattice, int64_t n, const int64
 olbgfx_status olbgfx_set_bouzidi_velocity(olbgfx_lattice* lattice, int64_t n, const int64_t* cells,
                                           int32_t value_type, const void* q, const void* velocity);
 
-/* ---- populations -----------------------------------------------------------/* ---- populations -----------------------------------------------------------
+/* ---- populations -----------------------------------------------------------
  * iniEquilibrium(indicator, rho, u) (core/blockLattice.hh:202-236,
  * dynamics/interface.h:111-121): f_bar_i = feq_i(rho, u) on selected cells.
  * sel: one byte per cell (NULL = all), rho: ncells, u: 3*ncells. */

@@ -97,6 +97,8 @@ def library() -> ctypes.CDLL:
     sigs = {
         "olbgfx_lattice_create": [ctypes.POINTER(_LatticeDesc), ctypes.POINTER(vp)],
         "olbgfx_lattice_destroy": [vp],
+        "olbgfx_device_count": [ctypes.POINTER(i32)],
+        "olbgfx_set_inflight_limit": [vp, u32],
         "olbgfx_lattice_ncells": [vp, ctypes.POINTER(ctypes.c_uint64)],
         "olbgfx_neighbor_distances": [vp, ctypes.POINTER(ctypes.c_int64)],
         "olbgfx_ring_shifts": [vp, ctypes.POINTER(ctypes.c_int64)],
@@ -323,6 +325,8 @@ class Lattice:
         if rho.size != self.ncells or u.size != 3 * self.ncells:
             raise ValueError("rho/u arrays have the wrong size")
         s = None if sel is None else np.ascontiguousarray(sel, dtype=np.uint8).reshape(-1)
+        if s is not None and s.size != self.ncells:
+            raise ValueError("the selection mask needs one entry per padded cell")
         _check(library().olbgfx_ini_equilibrium(self._h, _ptr(s), self._value, _ptr(rho), _ptr(u)))
 
     def get_populations(self) -> np.ndarray:

@@ -1612,6 +1612,7 @@ uint64_t magic_for(uint32_t d)
 
 }  // namespace
 
+
 struct olbgfx_lattice {
   olbgfx_lattice_desc desc{};
   int elem = 4;               // bytes per stored population
@@ -1666,6 +1667,11 @@ struct olbgfx_lattice {
   std::vector<uint8_t> host_mask;
   uint64_t version = 0;        // bumped by every change of dynamics, mask or fields
   bool shift_dev_valid = true; // device copy of the offsets matches the host mirror
+  // at most inflight_limit steps queued ahead of the host (0: no limit): an
+  // event per step in a ring, the host waits for the one of step t - limit
+  uint32_t inflight_limit = 0;
+  std::vector<hipEvent_t> inflight_events;
+  uint64_t inflight_count = 0;
 };
 
 namespace {
@@ -1857,6 +1863,14 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
     if (launch_post<S>(L, a, stream, true) != OLBGFX_OK) {
       return OLBGFX_ERR_HIP;
     }
+    if (L->inflight_limit) {
+      hipEvent_t& e = L->inflight_events[L->inflight_count % L->inflight_limit];
+      if (L->inflight_count >= L->inflight_limit) {
+        HIP_TRY(hipEventSynchronize(e));
+      }
+      HIP_TRY(hipEventRecord(e, stream));
+      ++L->inflight_count;
+    }
   }
   if (steps > 0) {
     L->stats_valid = L->stats_on;
@@ -2000,6 +2014,17 @@ olbgfx_status prepare_boundary(olbgfx_lattice* L)
   bool any = false;
   for (int k = 0; k < OLBGFX_MAX_DYNAMICS; ++k) {
     any = any || is_velocity_boundary(L->table[k].kind);
+  }
+  // the momenta of these kinds read the normal of every cell that holds
+  // them, padding included (computeRhoU, the post-processors' neighbours)
+  if (any && L->host_normals.empty()) {
+    for (size_t n = 0; n < L->count; ++n) {
+      if (is_velocity_boundary(L->table[L->host_mask[n]].kind)) {
+        return fail(OLBGFX_ERR_INVALID,
+                    "a cell holds a boundary kind that reads the discrete normal, but no normals were set "
+                    "(olbgfx_set_cell_normals)");
+      }
+    }
   }
   std::vector<uint32_t> edges, corners, fd;
   bool used = false;
@@ -2274,7 +2299,44 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->bzv_v);
   hipFree(L->stats_partial);
   hipFree(L->stats_out);
+  for (hipEvent_t e : L->inflight_events) {
+    hipEventDestroy(e);
+  }
   delete L;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_inflight_limit(olbgfx_lattice* L, uint32_t steps)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (steps > 64) {
+    return fail(OLBGFX_ERR_INVALID, "at most 64 steps in flight");
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  for (hipEvent_t e : L->inflight_events) {
+    HIP_TRY(hipEventDestroy(e));
+  }
+  L->inflight_events.assign(steps, nullptr);
+  for (uint32_t k = 0; k < steps; ++k) {
+    HIP_TRY(hipEventCreateWithFlags(&L->inflight_events[k], hipEventDisableTiming));
+  }
+  L->inflight_limit = steps;
+  L->inflight_count = 0;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_device_count(int32_t* count)
+{
+  if (!count) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    n = 0;
+  }
+  *count = n;
   return OLBGFX_OK;
 }
 
@@ -2432,6 +2494,12 @@ olbgfx_status set_bouzidi_list(olbgfx_lattice* L, int64_t n, const int64_t* cell
   }
   // a cell's update reads x_s[i] (x_s = x_b + c_i) and, for q > 1/2,
   // x_f[opp i] (x_f = x_b - c_i); neither may be written by another update
+  // of the same list, whose cells run concurrently.  The plain and the
+  // velocity list are not checked against each other: they run as two
+  // launches, the plain list first, which is the reference's own order (all
+  // cells of BouzidiPostProcessor, then all of BouzidiVelocityPostProcessor,
+  // in the one priority of core/blockPostProcessorMap.h), so a cell in both
+  // lists, or one list reading what the other wrote, sees what it sees there
   auto entry = [&](int64_t c) -> int64_t {
     const int64_t* it = std::lower_bound(cells, cells + n, c);
     return (it != cells + n && *it == c) ? it - cells : -1;

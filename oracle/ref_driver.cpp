@@ -179,6 +179,24 @@ void dump_normals(Dump& d, SuperGeometry<T,3>& G, int padding, std::initializer_
 // SuperVTMwriter3D output (io/superVtmWriter3D.hh) of the lattice at iT
 // into DIR/vtkData/: velocity, density, physVelocity and physPressure, once
 // compressed (the default) as NAME and once as plain binary as NAME_raw
+// relative L2 error of the physical strain rate over the fluid (material 1),
+// as testPoiseuille3DBGK.cpp:277-287 computes it
+template <typename DESCRIPTOR>
+double strain_error_l2(SuperLattice<T,DESCRIPTOR>& L, SuperGeometry<T,3>& G,
+                       const UnitConverter<T,DESCRIPTOR>& converter)
+{
+  int tmp[] = {int()};
+  T result[2] = {T(), T()}, resultSol[2] = {T(), T()};
+  CirclePoiseuilleStrainRate3D<T,DESCRIPTOR> sSol(converter, T(0.5));
+  SuperLatticePhysStrainRate3D<T,DESCRIPTOR> s(L, converter);
+  SuperLatticeFfromAnalyticalF3D<T,DESCRIPTOR> sSolLattice(sSol, L);
+  SuperL2Norm3D<T> sL2Norm(sSolLattice - s, G, 1);
+  SuperL2Norm3D<T> sSolL2Norm(sSolLattice, G, 1);
+  sL2Norm(result, tmp);
+  sSolL2Norm(resultSol, tmp);
+  return result[0] / resultSol[0];
+}
+
 template <typename DESCRIPTOR>
 void write_vtk(SuperLattice<T,DESCRIPTOR>& L, const UnitConverter<T,DESCRIPTOR>& converter,
                const std::string& dir, const std::string& name, int iT)
@@ -729,6 +747,7 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   uL2Norm(result, tmp);
   uSolL2Norm(resultSol, tmp);
   const double velocityErrorL2 = result[0] / resultSol[0];
+  const double strainErrorL2 = strain_error_l2(L, G, converter);
   double pressureErrorL2 = 0;
   const T p0phys = 4. * converter.getPhysViscosity() * maxVelocity * lx / (r * r);
   AnalyticalLinear3D<T,T> pressureSol(-p0phys / lx, 0, 0, p0phys);
@@ -744,31 +763,37 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
 
   auto& bg = G.getBlockGeometry(0);
   std::vector<T> usol, psol;
+  CirclePoiseuilleStrainRate3D<T,DESCRIPTOR> sSol(converter, r);
+  std::vector<T> ssol;
   for (int x = -padding; x < bg.getNx() + padding; ++x) {
     for (int y = -padding; y < bg.getNy() + padding; ++y) {
       for (int z = -padding; z < bg.getNz() + padding; ++z) {
-        T v[3] = {0, 0, 0}, pv[1] = {0};
+        T v[3] = {0, 0, 0}, pv[1] = {0}, sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (bg.get({x, y, z}) == 1) {
           Vector<T,3> physR;
           bg.getPhysR(physR, {x, y, z});
           uSol(v, physR.data());
           pressureSol(pv, physR.data());
+          sSol(sv, physR.data());
         }
         usol.insert(usol.end(), v, v + 3);
         psol.push_back(pv[0]);
+        ssol.insert(ssol.end(), sv, sv + 9);
       }
     }
   }
   char buf[768];
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
-                "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu",
+                "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu "
+                "strainErrorL2=%.17g dt=%.17g tau=%.17g",
                 bouzidi ? "bouzidi" : (bounceBack ? "bounceBack" : (interpolated ? "interpolated" : "local")),
                 forced ? 1 : 0, n, iT,
                 (double)omega, (double)force0,
                 (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
-                converter.getLatticeTime(maxPhysT));
+                converter.getLatticeTime(maxPhysT), strainErrorL2, (double)converter.getConversionFactorTime(),
+                (double)converter.getLatticeRelaxationTime());
   Dump d(out, block_header(L) + buf);
   d.array("materials", "i4", materials(G, padding));
   dump_normals(d, G, padding, {2, 3, 4});
@@ -780,6 +805,7 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
   d.array("energy", "f8", energy);
   d.array("usol", tname(), usol);
   d.array("psol", tname(), psol);
+  d.array("ssol", tname(), ssol);
   if (bouzidi) {
     // BOUZIDI_DISTANCE (-1: none) and BOUZIDI_VELOCITY of every cell,
     // population-major
@@ -881,20 +907,24 @@ int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
   uL2Norm(result, tmp);
   uSolL2Norm(resultSol, tmp);
   const double velocityErrorL2 = result[0] / resultSol[0];
+  const double strainErrorL2 = strain_error_l2(L, G, converter);
 
-  // analytic velocity at every padded cell (zero outside material 1)
+  // analytic velocity and strain rate at every padded cell (zero outside material 1)
   auto& bg = G.getBlockGeometry(0);
-  std::vector<T> usol;
+  std::vector<T> usol, ssol;
+  CirclePoiseuilleStrainRate3D<T,DESCRIPTOR> sSol(converter, r);
   for (int x = -padding; x < bg.getNx() + padding; ++x) {
     for (int y = -padding; y < bg.getNy() + padding; ++y) {
       for (int z = -padding; z < bg.getNz() + padding; ++z) {
-        T v[3] = {0, 0, 0};
+        T v[3] = {0, 0, 0}, sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (bg.get({x, y, z}) == 1) {
           Vector<T,3> physR;
           bg.getPhysR(physR, {x, y, z});
           uSol(v, physR.data());
+          sSol(sv, physR.data());
         }
         usol.insert(usol.end(), v, v + 3);
+        ssol.insert(ssol.end(), sv, sv + 9);
       }
     }
   }
@@ -904,12 +934,14 @@ int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
   char buf[768];
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g deltaT=%zu "
-                "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu origin=%.17g,%.17g,%.17g delta=%.17g",
+                "residuum=%.17g velocityErrorL2=%.17g max_steps=%zu origin=%.17g,%.17g,%.17g delta=%.17g "
+                "strainErrorL2=%.17g dt=%.17g tau=%.17g",
                 bouzidi ? "bouzidi" : "bounceBack", n, iT, (double)omega, (double)f[0],
                 (double)converter.getConversionFactorVelocity(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2,
                 converter.getLatticeTime(maxPhysT), (double)origin[0], (double)origin[1],
-                (double)origin[2], (double)delta);
+                (double)origin[2], (double)delta, strainErrorL2, (double)converter.getConversionFactorTime(),
+                (double)converter.getLatticeRelaxationTime());
   Dump d(out, block_header(L) + buf);
   d.array("materials", "i4", materials(G, padding));
   d.array("distances", "i8", distances(L));
@@ -917,6 +949,7 @@ int run_poiseuille(int n, const std::string& out, bool bouzidi = false)
   d.array("f1", tname(), populations(L));
   d.array("energy", "f8", energy);
   d.array("usol", tname(), usol);
+  d.array("ssol", tname(), ssol);
   if (bouzidi) {
     // the Bouzidi distances of every cell, population-major (-1: none)
     auto& b = L.getBlock(0);

@@ -259,6 +259,8 @@ def main() -> None:
                 checkpoint_fixtures()
             elif name == "bouzidi":
                 poiseuille_bouzidi()
+            elif name == "poiseuille_bounceback":
+                poiseuille_bounceback()
             elif name == "vtk":
                 vtk_fixtures()
             elif name == "poiseuille_local":
@@ -304,6 +306,12 @@ def main() -> None:
                      repr(P["magic"]), repr(P["fx"]), str(P["steps"]), path])
         save(f"porous16x12x12_{tag}", *read_dump(path), extra={"spheres": centres.tolist(), **P})
 
+    poiseuille_bounceback()
+    poiseuille_bouzidi()
+
+
+def poiseuille_bounceback() -> None:
+    """the forced bounceBack row of testPoiseuille3DBGK.cpp at N = 11"""
     path = os.path.join(RUN, "poiseuille.bin")
     run("olb_ref_d", ["poiseuille", "11", path])
     meta, arrays = read_dump(path)
@@ -321,10 +329,10 @@ def main() -> None:
         "f1_fluid": arrays["f1"].reshape(q, n)[:, fluid],
         "energy": arrays["energy"],
         "usol_fluid": arrays["usol"].reshape(n, 3)[fluid],
+        "ssol_fluid": arrays["ssol"].reshape(n, 9)[fluid],
     }
     np.savez_compressed(os.path.join(HERE, "poiseuille11_forced_bounceback_f64.npz"), **out)
     print("wrote poiseuille", meta)
-    poiseuille_bouzidi()
 
 
 def vtk_fixtures() -> None:
@@ -402,6 +410,7 @@ def poiseuille_rows(rows, threads: int = 4) -> None:
             "f1_wall": arrays["f1"].reshape(q, n)[:, wall],
             "energy": arrays["energy"],
             "usol_fluid": arrays["usol"].reshape(n, 3)[fluid],
+            "ssol_fluid": arrays["ssol"].reshape(n, 9)[fluid],
             "psol_fluid": arrays["psol"][fluid],
             "rho0": arrays["rho0"][keep],
             "u0": arrays["u0"].reshape(n, 3)[keep],
@@ -445,6 +454,7 @@ def poiseuille_bouzidi() -> None:
         "f1_bz": arrays["f1"].reshape(q, n)[:, bz_cells],
         "energy": arrays["energy"],
         "usol_fluid": arrays["usol"].reshape(n, 3)[fluid],
+        "ssol_fluid": arrays["ssol"].reshape(n, 9)[fluid],
         "bz_cells": bz_cells.astype(np.int64),
         "bz_q": np.ascontiguousarray(field[:, bz_cells].T),
         "field_cells": entries.astype(np.int64),
@@ -611,6 +621,7 @@ def poiseuille_bouzidi_nonforced(n: int = 11) -> None:
         "f1_fluid": arrays["f1"].reshape(19, nq)[:, fluid],
         "energy": arrays["energy"],
         "usol_fluid": arrays["usol"].reshape(nq, 3)[fluid],
+        "ssol_fluid": arrays["ssol"].reshape(nq, 9)[fluid],
         "psol_fluid": arrays["psol"][fluid],
         "rho0": arrays["rho0"][keep],
         "u0": arrays["u0"].reshape(nq, 3)[keep],

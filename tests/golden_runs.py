@@ -251,6 +251,67 @@ def velocity_error_l2(lat, meta, d) -> float:
     return float(np.sqrt(np.sum((sol - u_phys) ** 2) / np.sum(sol ** 2)))
 
 
+# testPoiseuille3DBGK.cpp:54-66: iT, velocity, pressure and strain-rate L2
+# errors and the error factor of each row (the tolerance is factor * 1e-6)
+KAT_TABLE = {
+    "poiseuille11_forced_bounceback_f64": (422, 0.158042, 0.0, 0.168242, 1),
+    "poiseuille11_forced_local_f64": (317, 0.020342, 0.0, 0.0165907, 1),
+    "poiseuille11_forced_interpolated_f64": (303, 0.016909, 0.0, 0.016009, 1),
+    "poiseuille11_forced_bouzidi_f64": (294, 0.019703, 0.0, 0.0232073, 1),
+    "poiseuille11_nonforced_bounceback_f64": (621, 0.1284974, 0.40862686, 0.3209910, 1),
+    "poiseuille11_nonforced_local_f64": (570, 0.05043437, 0.10581659, 0.05099981, 1),
+    "poiseuille11_nonforced_interpolated_f64": (570, 0.04976679, 0.10483063, 0.04959827, 1),
+    "poiseuille11_nonforced_bouzidi_f64": (572, 0.056544195, 0.070186678, 0.088491618, 1),
+}
+
+
+def strain_error_l2(lat, meta, d) -> float:
+    """relative L2 error of the physical strain rate over the fluid, as the
+    test computes it (testPoiseuille3DBGK.cpp:277-287) through
+    SuperLatticePhysStrainRate3D (functors/lattice/latticePhysStrainRate3D.hh:
+    S = -Pi omega c_s^-2 / (2 rho dt)): the momenta of the fluid's dynamics
+    give rho, u (with F/2 for the forced ones, momenta::Forced keeps the
+    bulk stress, dynamics/momenta/interface.h:288-296) and Pi^neq
+    (lbm.h:334-352, computed with that u); the analytic strain rate is the
+    reference's CirclePoiseuilleStrainRate3D at each cell (fixture)"""
+    from tests.helpers import C
+
+    sel = core_fluid(meta, d)
+    f = lat.get_populations()[:, d["fluid"][sel]].astype(np.float64)
+    c = np.asarray(C, dtype=np.float64)
+    rho = f.sum(0) + 1.0
+    u = (c.T @ f) / rho
+    if meta.get("forced", 1 if meta["force"] != 0 else 0):
+        u[0] += 0.5 * meta["force"]
+    omega = 1.0 / meta["tau"]
+    s = np.empty((9, f.shape[1]))
+    pairs = {(0, 0): [0], (0, 1): [1, 3], (0, 2): [2, 6], (1, 1): [4], (1, 2): [5, 7], (2, 2): [8]}
+    for (a, b), outs in pairs.items():
+        pi = (c[:, a] * c[:, b]) @ f - rho * u[a] * u[b]
+        if a == b:
+            pi -= (rho - 1.0) / 3.0
+        val = -pi * omega * 3.0 / rho / 2.0 / meta["dt"]
+        for k in outs:
+            s[k] = val
+    sol = d["ssol_fluid"][sel].T
+    return float(np.sqrt(np.sum((sol - s) ** 2) / np.sum(sol ** 2)))
+
+
+def assert_strain(lat, meta, d, name: str, tol_checker: float = 1e-9, table: bool = True) -> float:
+    """the strain-rate column of the known-answer test for one row: within
+    tol_checker of the reference built here (the fixture's strainErrorL2)
+    and, where table is set, within the row's errorFactor * 1e-6 of the
+    test's table (testPoiseuille3DBGK.cpp:287)"""
+    s = strain_error_l2(lat, meta, d)
+    row = KAT_TABLE[name]
+    print(f"{name}: strain-rate L2 error {s:.9g} (reference built here {meta['strainErrorL2']:.9g}, "
+          f"table {row[3]})", flush=True)
+    assert abs(s - meta["strainErrorL2"]) <= tol_checker
+    if table:
+        assert abs(s - row[3]) <= row[4] * 1e-6
+    return s
+
+
 def cavity_reduced(lat, meta, d):
     """mid-plane velocity, sample populations and fluid sums of a cavity run,
     in the layout of the cavity3d_res64 fixture"""

@@ -12,7 +12,7 @@ import pytest
 from openlb_amd import bouzidi
 from oracle.oracle import OracleLattice
 from tests.golden_runs import (bouzidi_entries, bouzidi_field, build, core_fluid, load, padded_shape,
-                               poiseuille_wall, value_tracer_run, velocity_error_l2)
+                               poiseuille_wall, value_tracer_run, velocity_error_l2, assert_strain)
 from tests.helpers import rel_err_unshifted
 
 NAME = "poiseuille11_forced_bouzidi_f64"
@@ -118,6 +118,7 @@ def test_oracle_poiseuille_bouzidi_known_answer():
     assert it == 294 == meta["iT"]
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, NAME)
     print("oracle bouzidi velocity error", err)
     assert abs(err - 0.019703) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9

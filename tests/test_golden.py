@@ -9,7 +9,7 @@ import pytest
 
 from openlb_amd import cases
 from oracle.oracle import OracleLattice
-from tests.golden_runs import (build, compared_cells, core_fluid, load, local_velocity_boundary_errors,
+from tests.golden_runs import (assert_strain, build, compared_cells, core_fluid, load, local_velocity_boundary_errors,
                                value_tracer_run, velocity_error_l2)
 from tests.helpers import T as T_W
 from tests.helpers import rel_err_unshifted
@@ -77,6 +77,7 @@ def test_oracle_poiseuille_known_answer():
     assert it == 422 == meta["iT"]
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_bounceback_f64")
     assert abs(err - 0.158042) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9
     sel = core_fluid(meta, d)
@@ -192,6 +193,7 @@ def test_oracle_poiseuille_nonforced_local_known_answer():
     assert it == 570 == meta["iT"]
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_local_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("velocity", err, "pressure", perr)
     assert abs(err - 0.05043437) <= 1e-6 and abs(err - meta["velocityErrorL2"]) <= 1e-9
@@ -215,6 +217,7 @@ def test_oracle_poiseuille_forced_local_known_answer():
     assert it == 317 == meta["iT"]
     np.testing.assert_allclose(energy[:4], d["energy"][:4], rtol=1e-12)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_local_f64", tol_checker=3e-6)
     print("velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - 0.020342) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 3e-6
@@ -276,6 +279,7 @@ def test_oracle_poiseuille_nonforced_interpolated_known_answer():
     assert it == 570 == meta["iT"]
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_interpolated_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("velocity", err, "pressure", perr)
     assert abs(err - 0.04976679) <= 1e-6 and abs(err - meta["velocityErrorL2"]) <= 1e-9
@@ -300,6 +304,7 @@ def test_oracle_poiseuille_forced_interpolated_known_answer():
     assert it == 303
     np.testing.assert_allclose(energy[:5], d["energy"][:5], rtol=1e-12)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_interpolated_f64", tol_checker=5e-5)
     print("velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - 0.016909) <= 1e-6
 
@@ -323,6 +328,7 @@ def test_oracle_poiseuille_nonforced_bouzidi_matches_reference():
     assert it == meta["iT"] == 572
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_bouzidi_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("velocity", err, "pressure", perr)
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
@@ -344,6 +350,7 @@ def test_oracle_poiseuille_nonforced_bounceback_known_answer():
     assert it == meta["iT"] == 621
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_bounceback_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("velocity", err, "pressure", perr)
     assert abs(err - 0.1284974) <= 1e-6 and abs(err - meta["velocityErrorL2"]) <= 1e-9

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from openlb_amd import Lattice
-from tests.golden_runs import build, compared_cells, core_fluid, load, value_tracer_run, velocity_error_l2
+from tests.golden_runs import assert_strain, build, compared_cells, core_fluid, load, value_tracer_run, velocity_error_l2
 from tests.helpers import rel_err_unshifted
 
 pytestmark = pytest.mark.gpu
@@ -58,6 +58,7 @@ def test_gpu_poiseuille_known_answer():
     assert it == 422
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_bounceback_f64")
     print("poiseuille gpu velocity error", err)
     assert abs(err - 0.158042) <= 1e-6
     sel = core_fluid(meta, d)
@@ -156,6 +157,7 @@ def test_gpu_poiseuille_bouzidi_known_answer():
     assert it == 294
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_bouzidi_f64")
     print("poiseuille bouzidi gpu velocity error", err)
     assert abs(err - 0.019703) <= 1e-6
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9
@@ -281,6 +283,7 @@ def test_gpu_poiseuille_nonforced_local_known_answer():
     assert it == 570
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_local_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("nonforced local gpu velocity", err, "pressure", perr)
     assert abs(err - 0.05043437) <= 1e-6 and abs(perr - 0.10581659) <= 1e-6
@@ -301,6 +304,7 @@ def test_gpu_poiseuille_forced_local_known_answer():
     it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
     assert it == 317
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_local_f64", tol_checker=1e-6)
     print("forced local gpu velocity", err, "reference", meta["velocityErrorL2"])
     assert abs(err - meta["velocityErrorL2"]) <= 1e-6
     assert abs(err - 0.020342) <= 3e-6
@@ -366,6 +370,7 @@ def test_gpu_poiseuille_nonforced_interpolated_known_answer():
     assert it == 570
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_interpolated_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("nonforced interpolated gpu velocity", err, "pressure", perr)
     assert abs(err - 0.04976679) <= 1e-6 and abs(perr - 0.10483063) <= 1e-6
@@ -389,6 +394,7 @@ def test_gpu_poiseuille_forced_interpolated_known_answer():
     lat = build(Lattice, meta, d, "f64")
     it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_forced_interpolated_f64", tol_checker=1e-4, table=False)
     print("forced interpolated gpu", it, "velocity", err, "reference", meta["iT"], meta["velocityErrorL2"])
     assert it == meta["iT"] == 302
     assert abs(err - meta["velocityErrorL2"]) <= 1e-5
@@ -429,6 +435,7 @@ def test_gpu_poiseuille_nonforced_bouzidi_matches_reference():
     assert it == 572
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_bouzidi_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("nonforced bouzidi gpu velocity", err, "pressure", perr)
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
@@ -451,6 +458,7 @@ def test_gpu_poiseuille_nonforced_bounceback_known_answer():
     assert it == 621
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, "poiseuille11_nonforced_bounceback_f64")
     perr = _pressure_error_l2(lat, meta, d)
     print("nonforced bounceback gpu velocity", err, "pressure", perr)
     assert abs(err - 0.1284974) <= 1e-6 and abs(perr - 0.40862686) <= 1e-6
