@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--porous-steps", type=int, default=500)
     ap.add_argument("--no-slab", action="store_true")
     ap.add_argument("--no-les", action="store_true")
+    ap.add_argument("--no-probes", action="store_true")
+    ap.add_argument("--no-openlb-cases", action="store_true")
     ap.add_argument("--tune-cache", default="",
                     help="autotuner cache file; launch choices found there are reused without trials")
     ap.add_argument("--calib-copy", action="store_true",
@@ -315,22 +317,48 @@ def time_slab(torch, steps, storage="f32", shape=(64, 512, 512)):
             "launch": {k: tuned[k] for k in ("block", "waves", "order")}}
 
 
-def copy_bandwidth(torch, nbytes=4 << 30, reps=10):
-    """measured device copy bandwidth (read + write bytes / time)"""
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e.record()
-    torch.cuda.synchronize()
-    ms = s.elapsed_time(e) / reps
-    del a, b
-    torch.cuda.empty_cache()
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+def probe_ceilings(size, storages, orders):
+    """bandwidth of the step kernel's access patterns on the same lattice
+    (include/olbgfx_probe.h): a plain 16-byte copy of the rings, and the 19
+    PS rings read only, written only, and read-modified-written in place
+    with the step kernel's indexing, block shape and (tuned) block order,
+    after one step of ring shifts"""
+    from openlb_amd import probe_bandwidth
+
+    out = {}
+    for st in storages:
+        order = orders.get(st, -1)
+        out[st] = {pat: round(probe_bandwidth((size, size, size), 3, st, pat, 1, order, 10), 1)
+                   for pat in ("copy", "ring_read", "ring_write", "ring_inplace")}
+        out[st]["block_order"] = order
+    return out
+
+
+def openlb_cases(resolution=100, steps=1000):
+    """OpenLB's own cavity3dBenchmark built unchanged on the GPU_GFX950
+    platform (openlb_amd/platform/bin): the MLUPs it prints itself, the
+    metric of the reference's GPU CI (.ci/benchmark_cuda.sh:15-20), for its
+    default LocalVelocity walls in FP32 and FP16 storage and its bounce-back
+    variant"""
+    binary = os.path.join(ROOT, "openlb_amd", "platform", "bin")
+    runs = {"cavity3d_f32": ("cavity3d", "f32"), "cavity3d_f16": ("cavity3d", "f16"),
+            "cavity3d_bounceback_f32": ("cavity3d_bounceback", "f32")}
+    out = {}
+    for key, (case, storage) in runs.items():
+        path = os.path.join(binary, case)
+        if not os.path.exists(path):
+            continue
+        env = dict(os.environ, OMP_NUM_THREADS="4", OLBGFX_STORAGE=storage)
+        try:
+            with tempfile.TemporaryDirectory() as scratch:
+                text = subprocess.run([path, "--RESOLUTION", str(resolution), "--TIME_STEPS", str(steps),
+                                       "--VTK_ENABLED", "0"], env=env, cwd=scratch, capture_output=True,
+                                      text=True, timeout=300, check=True).stdout
+            last = text.strip().splitlines()[-1].split(",")
+            out[key] = {"mlups": float(last[-1]), "resolution": resolution, "time_steps": steps}
+        except Exception as exc:
+            sys.stderr.write(f"{case} ({storage}) failed: {exc}\n")
+    return out
 
 
 def cpu_baseline(size, steps):
@@ -429,9 +457,20 @@ def main():
     if acc:
         print("# accuracy vs f64: " + ", ".join(f"{k} L2 {acc[k]['velocity_l2_vs_f64']:.2e}"
                                                 for k in ("f32", "f16", "bf16")), file=sys.stderr)
-    copy_gbps = copy_bandwidth(torch)
     head = res.get("f32") or next(iter(res.values()))
     head_storage = "f32" if "f32" in res else storages[0]
+    probes = None
+    if not args.no_probes:
+        torch.cuda.empty_cache()
+        probes = probe_ceilings(args.size, [head_storage] + (["f16"] if "f16" in res and head_storage != "f16" else []),
+                                {k: v["launch"]["order"] for k, v in res.items()})
+        print("# probes: " + "; ".join(f"{k} " + ", ".join(f"{p} {v[p]:.0f}" for p in
+                                                           ("copy", "ring_read", "ring_write", "ring_inplace"))
+                                       for k, v in probes.items()) + " GB/s", file=sys.stderr)
+    cases = None if args.no_openlb_cases else openlb_cases()
+    if cases:
+        print("# OpenLB cavity3dBenchmark on GPU_GFX950, RESOLUTION 100, 1000 steps: "
+              + ", ".join(f"{k} {v['mlups']:.0f} MLUPs" for k, v in cases.items()), file=sys.stderr)
     cells = args.size ** 3
     alg_bytes = head["bytes_per_cell"] * cells
     achieved = alg_bytes / (head["kernel_ms"] * 1e-3) / 1e9
@@ -445,9 +484,15 @@ def main():
         "traffic": traffic,
         "kernel": f"step_kernel<{head_storage}, BGK>",
         "bytes_per_cell": head["bytes_per_cell"],
-        "measured_copy_GBps": round(copy_gbps, 1),
-        "frac_of_measured_copy": round(achieved / copy_gbps, 4),
     }
+    if probes:
+        p = probes[head_storage]
+        roofline.update({
+            "measured_copy_GBps": p["copy"],
+            "frac_of_measured_copy": round(achieved / p["copy"], 4),
+            "pattern_ceiling_GBps": p["ring_inplace"],
+            "frac_of_pattern_ceiling": round(achieved / p["ring_inplace"], 4),
+        })
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_size, args.cpu_steps)
     line = {
         "metric": "D3Q19 GLUPS & %HBM roofline at 512^3 (FP32 vs FP16 store); ensemble steps/s",
@@ -475,6 +520,8 @@ def main():
         "slab": slab,
         "smagorinsky": les,
         "accuracy": acc,
+        "probes_GBps": probes,
+        "openlb_cavity3d_benchmark": cases,
         "roofline": roofline,
         "cpu_baseline": base,
     }

@@ -98,6 +98,8 @@ def library() -> ctypes.CDLL:
         "olbgfx_lattice_create": [ctypes.POINTER(_LatticeDesc), ctypes.POINTER(vp)],
         "olbgfx_lattice_destroy": [vp],
         "olbgfx_device_count": [ctypes.POINTER(i32)],
+        "olbgfx_probe_bandwidth": [ctypes.POINTER(_LatticeDesc), i32, u32, i32, u32, vp,
+                                   ctypes.POINTER(ctypes.c_double)],
         "olbgfx_set_inflight_limit": [vp, u32],
         "olbgfx_lattice_ncells": [vp, ctypes.POINTER(ctypes.c_uint64)],
         "olbgfx_neighbor_distances": [vp, ctypes.POINTER(ctypes.c_int64)],
@@ -462,6 +464,24 @@ class CellList:
             self.close()
         except Exception:
             pass
+
+
+PROBES = {"copy": 0, "ring_read": 1, "ring_write": 2, "ring_inplace": 3}
+
+
+def probe_bandwidth(extent, padding: int, storage: str, pattern: str, shift_steps: int = 1,
+                    block_order: int = -1, reps: int = 10, stream: int | None = None) -> float:
+    """GB/s of one access pattern of the step kernel (include/olbgfx_probe.h)
+    on a lattice of that shape and storage: a plain 16-byte copy of the rings,
+    the 19 rings read, written, or read-modified-written in place"""
+    d = _LatticeDesc()
+    d.nx, d.ny, d.nz = (int(v) for v in extent)
+    d.padding = int(padding)
+    d.storage = STORE[storage]
+    out = ctypes.c_double()
+    _check(library().olbgfx_probe_bandwidth(ctypes.byref(d), PROBES[pattern], int(shift_steps),
+                                            int(block_order), int(reps), stream, ctypes.byref(out)))
+    return out.value
 
 
 def device_buffer(nbytes: int) -> "HaloBuffer":

@@ -72,10 +72,20 @@ def build_oracle(with_reference: bool = True) -> None:
         _run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
 
 
+def build_openlb_cases() -> None:
+    """OpenLB's own cases, unchanged, on Platform::GPU_GFX950 (needs the
+    OpenLB sources; the binaries travel in openlb_amd/platform/bin)"""
+    from openlb_amd.platform import build_cases
+
+    if os.path.isdir(os.path.join(build_cases.OLB_ROOT, "src")):
+        build_cases.main([])
+
+
 def build_all() -> None:
     build_library()
     build_host_examples()
     build_oracle()
+    build_openlb_cases()
 
 
 if __name__ == "__main__":
