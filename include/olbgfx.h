@@ -248,6 +248,18 @@ olbgfx_status olbgfx_get_statistics(olbgfx_lattice* lattice, double out[4]);
  * core/platform/gpu/hip/statistics.hh:79-115.  Synchronises. */
 olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* lattice, double out[8], void* stream);
 
+/* The same reduction for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  It runs the kernel of
+ * olbgfx_reduce_moments into a fixed device array of the library (4 MiB per
+ * device, held until the process ends) instead of a temporary allocation,
+ * and returns the same values bit for bit.  Lattices of more than 2^24 core
+ * cells (65536 blocks of 256) are refused with OLBGFX_ERR_INVALID.  It calls
+ * no hipMalloc or hipFree.  Calls within one process are serialised on a
+ * lock, since every call on a device shares that device's array, which
+ * keeps the last call's partial sums.  Synchronises. */
+olbgfx_status olbgfx_reduce_moments_noalloc(olbgfx_lattice* lattice, double out[8], void* stream);
+
 /* density and velocity of every padded cell through the momenta of its
  * dynamics (Cell::computeRhoU, dynamics/momenta/elements.h): NoDynamics
  * 1 and 0, BounceBack its RHO (wall_rho) and 0, BounceBackVelocity its RHO

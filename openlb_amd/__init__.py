@@ -118,6 +118,7 @@ def library() -> ctypes.CDLL:
         "olbgfx_set_statistics": [vp, i32],
         "olbgfx_get_statistics": [vp, ctypes.POINTER(ctypes.c_double)],
         "olbgfx_reduce_moments": [vp, ctypes.POINTER(ctypes.c_double), vp],
+        "olbgfx_reduce_moments_noalloc": [vp, ctypes.POINTER(ctypes.c_double), vp],
         "olbgfx_compute_rho_u": [vp, i32, vp, vp],
         "olbgfx_set_launch": [vp, u32, u32],
         "olbgfx_set_block_order": [vp, i32],

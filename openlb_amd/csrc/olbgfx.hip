@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -1582,6 +1583,15 @@ __global__ void __launch_bounds__(256) k_moments(const S* pop, uint64_t stride, 
 }
 
 // ---------------------------------------------------------------------------
+// fixed scratch of olbgfx_reduce_moments_noalloc: one 8-double partial per
+// 256-cell block of k_moments, for lattices of up to 65536 blocks (2^24 core
+// cells); 4 MiB on every device the library runs on, held until the process
+// ends.  Host code never passes `scratch` itself to a kernel (in host code
+// the name is a stand-in, not a device address): it passes the address
+// hipGetSymbolAddress returns for the current device.
+constexpr uint64_t kNoAllocBlocks = 65536;
+
+// ---------------------------------------------------------------------------
 // host side
 
 thread_local std::string g_error;
@@ -1612,6 +1622,36 @@ uint64_t magic_for(uint32_t d)
 
 }  // namespace
 
+// the array itself (declared here, at file scope with internal linkage)
+static __device__ __align__(256) double scratch[8 * kNoAllocBlocks];
+
+namespace {
+
+// per-device address of `scratch`, looked up on the first no-alloc call on
+// that device with the device current; the host copy of the partials
+constexpr int kNoAllocMaxDevices = 64;
+double* g_noalloc_dev[kNoAllocMaxDevices] = {};
+double g_noalloc_host[8 * kNoAllocBlocks];
+std::mutex g_noalloc_mutex;
+
+// caller holds g_noalloc_mutex
+olbgfx_status noalloc_scratch(double** ptr)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    return fail(OLBGFX_ERR_INVALID, "device ordinal beyond the no-alloc scratch table");
+  }
+  if (!g_noalloc_dev[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_dev[dev] = (double*)p;
+  }
+  *ptr = g_noalloc_dev[dev];
+  return OLBGFX_OK;
+}
+
+}  // namespace
 
 struct olbgfx_lattice {
   olbgfx_lattice_desc desc{};
@@ -2794,6 +2834,58 @@ olbgfx_status olbgfx_reduce_moments(olbgfx_lattice* L, double out[8], void* stre
   }
   hipFree(partial);
   return st;
+}
+
+olbgfx_status olbgfx_reduce_moments_noalloc(olbgfx_lattice* L, double out[8], void* stream)
+{
+  if (check(L) != OLBGFX_OK || !out) {
+    return fail(OLBGFX_ERR_INVALID, "null argument");
+  }
+  const olbgfx_lattice_desc& D = L->desc;
+  const uint64_t ncore = (uint64_t)D.nx * D.ny * D.nz;
+  const uint64_t nb = (ncore + 255) / 256;
+  if (nb > kNoAllocBlocks) {
+    return fail(OLBGFX_ERR_INVALID,
+                "olbgfx_reduce_moments_noalloc: the lattice has more than 16777216 core cells "
+                "(65536 blocks of 256), the size of its fixed device scratch; use olbgfx_reduce_moments");
+  }
+  // one call at a time per process: the device scratch and the host copy of
+  // the partials are shared by every lattice
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  double* partial = nullptr;
+  if (noalloc_scratch(&partial) != OLBGFX_OK) {
+    return OLBGFX_ERR_HIP;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipStreamSynchronize(s));
+  if (sync_shifts(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_HIP;
+  }
+  // k_moments writes all 8 slots of every block it runs, so nothing left in
+  // the scratch by an earlier call is read
+  olbgfx_status st = dispatch_storage(L, [&](auto* tag) -> olbgfx_status {
+    using S = std::remove_pointer_t<decltype(tag)>;
+    k_moments<S><<<(uint32_t)nb, 256, 0, s>>>((const S*)L->pop, L->stride, L->count, L->shift_dev, L->mask,
+                                              L->kinds_dev, D.nx, D.ny, D.nz, D.padding, L->py, L->pz,
+                                              partial);
+    HIP_TRY(hipGetLastError());
+    return OLBGFX_OK;
+  });
+  if (st != OLBGFX_OK) {
+    return st;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess ||
+      hipMemcpy(g_noalloc_host, partial, 8 * sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost) != hipSuccess) {
+    return fail(OLBGFX_ERR_HIP, "moment reduction failed");
+  }
+  // the same order of sums as olbgfx_reduce_moments, so both agree bit for bit
+  double o[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t b = 0; b < nb; ++b) {
+    for (int k = 0; k < 6; ++k) o[k] += g_noalloc_host[8 * b + k];
+    o[6] = std::max(o[6], g_noalloc_host[8 * b + 6]);
+  }
+  for (int k = 0; k < 8; ++k) out[k] = o[k];
+  return OLBGFX_OK;
 }
 
 olbgfx_status olbgfx_compute_rho_u(olbgfx_lattice* L, int32_t vt, void* rho, void* u)
