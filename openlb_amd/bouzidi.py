@@ -210,3 +210,81 @@ def entries(q: np.ndarray, padding: int, shape):
                     raise ValueError("two Bouzidi cells read each other's updates")
     keep = (qq >= 0).any(axis=0)
     return cells[keep].astype(np.int64), np.ascontiguousarray(qq[:, keep].T)
+
+
+class CirclePoiseuille3D:
+    """CirclePoiseuille3D(center, normal, maxVelocity, radius)
+    (functors/analytical/frameChangeF3D.hh:353-356), i.e. CirclePowerLaw3D
+    with n = 1 (:166-169, 237-243), FP64: velocity component d is
+    maxVelocity * normal_d * (1 - (r_d / radius)^2), r_d the distance from
+    the centre in the two coordinates other than d (a pipe along a
+    coordinate axis); the normal is normalised."""
+
+    def __init__(self, center, normal, max_velocity: float, radius: float, scale: float = 1.0):
+        n = np.asarray(normal, dtype=np.float64)
+        self.center = np.asarray(center, dtype=np.float64)
+        self.normal = n / math.sqrt(float(n @ n))
+        self.max_velocity = float(max_velocity)
+        self.radius = float(radius)
+        self.scale = float(scale)
+
+    def __call__(self, x: np.ndarray) -> np.ndarray:
+        """velocities (..., 3) at points x (..., 3)"""
+        x = np.asarray(x, dtype=np.float64)
+        dx = x - self.center
+        out = np.empty_like(x)
+        for d in range(3):
+            a, b = [k for k in range(3) if k != d]
+            r = np.sqrt(dx[..., a] * dx[..., a] + dx[..., b] * dx[..., b])
+            out[..., d] = (self.scale * self.max_velocity * self.normal[d]) * (1.0 - (r / self.radius) ** 2)
+        return out
+
+
+def wall_velocities(q: np.ndarray, materials: np.ndarray, padding: int, origin, delta: float, velocity,
+                    solid: int) -> np.ndarray:
+    """the BOUZIDI_VELOCITY field (19 x padded cells, 0 where unset) that
+    setBouzidiVelocity (boundary/setBouzidiBoundary.h:380-420) writes for the
+    cells of material `solid` and the analytical velocity `velocity`
+    (a callable on points (..., 3)): for every cell x_s of the material,
+    padding included, and direction c_i with x_b = x_s + c_i inside the block
+    and a distance q_b[opp i] >= 0, the wall point is
+    x_b + q_b[opp i] dx c_opp(i) and the coefficient c_opp(i) . u(wall point)"""
+    m = np.asarray(materials)
+    P = np.array(m.shape)
+    v = np.zeros_like(q)
+    solids = np.argwhere(m == solid)
+    for i in range(1, Q):
+        o = OPP[i]
+        bp = solids + C[i]
+        inside = np.all((bp >= 0) & (bp < P), axis=1)
+        bp = bp[inside]
+        cell = np.ravel_multi_index(bp.T, m.shape)
+        qb = q[o, cell]
+        on = qb >= 0
+        bp, cell, qb = bp[on], cell[on], qb[on]
+        phys = np.asarray(origin, dtype=np.float64)[None, :] + (bp - padding).astype(np.float64) * delta
+        wall = phys + (qb * delta)[:, None] * C[o].astype(np.float64)[None, :]
+        u = velocity(wall)
+        v[o, cell] = u @ C[o].astype(np.float64)
+    return v
+
+
+def processor_entries(q: np.ndarray, materials: np.ndarray, padding: int, solid: int):
+    """the cells that run the post-processor setBouzidiBoundary attached for
+    the solid material `solid` (setBouzidiBoundary.h:290-330): a core cell
+    runs it when one of its distances points at a cell of that material,
+    and then applies all of its distances (the processor reads the whole
+    BOUZIDI_DISTANCE field of its cell); as entries() returns them"""
+    m = np.asarray(materials)
+    shape = m.shape
+    count = int(np.prod(shape))
+    mats = m.reshape(-1)
+    step = np.array([shape[1] * shape[2], shape[2], 1], dtype=np.int64)
+    d = C @ step
+    owner = np.zeros(count, dtype=bool)
+    for j in range(1, Q):
+        cells = np.flatnonzero(q[j] >= 0)
+        target = cells + d[j]
+        ok = (target >= 0) & (target < count)
+        owner[cells[ok][mats[target[ok]] == solid]] = True
+    return entries(np.where(owner[None, :], q, -1.0), padding, shape)

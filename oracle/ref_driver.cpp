@@ -782,18 +782,21 @@ int run_poiseuille_local(int n, const std::string& out, bool forced, int fixedSt
       }
     }
   }
-  char buf[768];
+  const auto& cuboid0 = G.getCuboidDecomposition().get(0);
+  const Vector<T,3> origin = cuboid0.getOrigin();
+  char buf[896];
   std::snprintf(buf, sizeof(buf),
                 " case=poiseuille boundary=%s forced=%d n=%d iT=%d omega=%.17g force=%.17g u_factor=%.17g "
                 "p_factor=%.17g deltaT=%zu residuum=%.17g velocityErrorL2=%.17g pressureErrorL2=%.17g max_steps=%zu "
-                "strainErrorL2=%.17g dt=%.17g tau=%.17g",
+                "strainErrorL2=%.17g dt=%.17g tau=%.17g origin=%.17g,%.17g,%.17g delta=%.17g",
                 bouzidi ? "bouzidi" : (bounceBack ? "bounceBack" : (interpolated ? "interpolated" : "local")),
                 forced ? 1 : 0, n, iT,
                 (double)omega, (double)force0,
                 (double)converter.getConversionFactorVelocity(), (double)converter.getConversionFactorPressure(),
                 converter.getLatticeTime(physInterval), (double)residuum, velocityErrorL2, pressureErrorL2,
                 converter.getLatticeTime(maxPhysT), strainErrorL2, (double)converter.getConversionFactorTime(),
-                (double)converter.getLatticeRelaxationTime());
+                (double)converter.getLatticeRelaxationTime(), (double)origin[0], (double)origin[1],
+                (double)origin[2], (double)cuboid0.getDeltaR());
   Dump d(out, block_header(L) + buf);
   d.array("materials", "i4", materials(G, padding));
   dump_normals(d, G, padding, {2, 3, 4});

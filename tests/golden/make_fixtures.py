@@ -591,19 +591,9 @@ def poiseuille_bouzidi_nonforced(n: int = 11) -> None:
     count = int(np.prod(shape))
     q = np.asarray(arrays["bouzidi"]).reshape(19, count)
     v = np.asarray(arrays["bouzidi_velocity"]).reshape(19, count)
-    step = np.array([shape[1] * shape[2], shape[2], 1], dtype=np.int64)
-    d = bz.C @ step
     lists = {}
     for solid, key in ((2, "bz"), (3, "bzv")):
-        owner = np.zeros(count, dtype=bool)
-        for j in range(1, 19):
-            has = q[j] >= 0
-            cells = np.flatnonzero(has)
-            target = cells + d[j]
-            ok = (target >= 0) & (target < count)
-            owner[cells[ok][mats[target[ok]] == solid]] = True
-        field = np.where(owner[None, :], q, -1.0)
-        cells, qq = bz.entries(field, p, shape)
+        cells, qq = bz.processor_entries(q, mats.reshape(shape), p, solid)
         lists[key + "_cells"] = cells.astype(np.int64)
         lists[key + "_q"] = qq
         if key == "bzv":

@@ -11,8 +11,8 @@ import pytest
 
 from openlb_amd import bouzidi
 from oracle.oracle import OracleLattice
-from tests.golden_runs import (bouzidi_entries, bouzidi_field, build, core_fluid, load, padded_shape,
-                               poiseuille_wall, value_tracer_run, velocity_error_l2, assert_strain)
+from tests.golden_runs import (bouzidi_entries, bouzidi_field, build, core_fluid, load, nonforced_bouzidi_lists,
+                               padded_shape, poiseuille_wall, value_tracer_run, velocity_error_l2, assert_strain)
 from tests.helpers import rel_err_unshifted
 
 NAME = "poiseuille11_forced_bouzidi_f64"
@@ -137,3 +137,69 @@ def test_oracle_poiseuille_bouzidi_with_own_distances():
     it, _ = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
     assert it == 294
     assert abs(velocity_error_l2(lat, meta, d) - 0.019703) <= 1e-6
+
+
+def test_circle_poiseuille_profile_closed_form():
+    """CirclePoiseuille3D along x: u_x = u_max (1 - r^2 / R^2) in the yz
+    distance from the axis, u_max on the axis, 0 on the wall, negative
+    outside, the other components 0; the profile along y or z rotates with
+    the normal"""
+    prof = bouzidi.CirclePoiseuille3D([2.0, 0.5, 0.5], [2.0, 0.0, 0.0], 0.04, 0.5)
+    pts = np.array([[0.3, 0.5, 0.5], [1.7, 1.0, 0.5], [0.0, 0.5, 0.0], [5.0, 0.5 + 0.3, 0.5 + 0.4],
+                    [1.0, 0.5 + 0.6, 0.5], [1.0, 0.5 + 0.1, 0.5 - 0.2]])
+    u = prof(pts)
+    r2 = (pts[:, 1] - 0.5) ** 2 + (pts[:, 2] - 0.5) ** 2
+    np.testing.assert_allclose(u[:, 0], 0.04 * (1 - r2 / 0.25), rtol=1e-14, atol=1e-16)
+    assert u[0, 0] == 0.04 and abs(u[1, 0]) < 1e-16 and abs(u[3, 0]) < 1e-16 and u[4, 0] < 0
+    assert np.all(u[:, 1:] == 0)
+    along_y = bouzidi.CirclePoiseuille3D([0.5, 0.0, 0.5], [0.0, -1.0, 0.0], 0.04, 0.5)
+    assert along_y(np.array([[0.5, 3.0, 0.5]]))[0, 1] == -0.04
+
+
+@pytest.mark.parametrize("name", ["poiseuille11_nonforced_bouzidi_f64", "poiseuille21_nonforced_bouzidi_f64"])
+def test_nonforced_bouzidi_lists_match_reference(name):
+    """the two post-processor lists of the non-forced bouzidi rows from this
+    package alone (setBouzidiBoundary's distances for the wall and the inflow
+    disc, setBouzidiVelocity's coefficients with the test's Poiseuille
+    profile) against the ones built from the reference's own
+    BOUZIDI_DISTANCE and BOUZIDI_VELOCITY fields: the same cells, distances
+    within a few ulps (the reference contracts a*b+c into FMAs) and wall
+    coefficients within 4 ulps of their 0.05-0.09 magnitude.  A coefficient
+    taken along c_i instead of c_opp(i), at x_b instead of the wall point,
+    or from the physical instead of the lattice velocity fails here."""
+    meta, d = load(name)
+    (bc, bq), (vc, vq, vv) = nonforced_bouzidi_lists(meta, d)
+    np.testing.assert_array_equal(bc, d["bz_cells"])
+    np.testing.assert_array_equal(vc, d["bzv_cells"])
+    assert np.max(np.abs(bq - d["bz_q"])) <= 5e-15
+    assert np.max(np.abs(vq - d["bzv_q"])) <= 5e-15
+    assert np.max(np.abs(vv - d["bzv_v"])) <= 4e-17
+    assert (d["bzv_v"] != 0).sum() > 400 and np.all((vv == 0) == (d["bzv_v"] == 0))
+
+
+def test_oracle_poiseuille21_nonforced_bouzidi_known_answer():
+    """testPoiseuille3DBGK.cpp:64 (non-forced, bouzidi, N = 21) from the
+    lists this package computes: the test's iT = 1400; velocity, pressure
+    and strain-rate L2 errors within 1e-9 of the reference built here and
+    within 1e-6 of the table's 0.01624583, 0.024478267 and 0.046465458;
+    energies within 1e-10 and fluid populations within 1e-12 of the
+    reference's run"""
+    from tests.test_golden import _pressure_error_l2
+
+    name = "poiseuille21_nonforced_bouzidi_f64"
+    meta, d = load(name)
+    (bc, bq), (vc, vq, vv) = nonforced_bouzidi_lists(meta, d)
+    lat = build(OracleLattice, meta, d, "f64")
+    lat.set_bouzidi(bc, bq)
+    lat.set_bouzidi_velocity(vc, vq, vv)
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == meta["iT"] == 1400
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    err = velocity_error_l2(lat, meta, d)
+    perr = _pressure_error_l2(lat, meta, d)
+    assert_strain(lat, meta, d, name)
+    print("N = 21 velocity", err, "pressure", perr)
+    assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
+    assert abs(err - 0.01624583) <= 1e-6 and abs(perr - 0.024478267) <= 1e-6
+    sel = core_fluid(meta, d)
+    assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12

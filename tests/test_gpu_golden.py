@@ -419,27 +419,36 @@ def test_gpu_poiseuille_nonforced_interpolated_storage(storage):
     assert err <= (1e-5 if storage == "f32" else 2e-3)
 
 
-def test_gpu_poiseuille_nonforced_bouzidi_matches_reference():
-    """testPoiseuille3DBGK.cpp:63 (non-forced, bouzidi, N = 11: Bouzidi wall,
-    velocity Bouzidi inflow, InterpolatedPressure outflow) on the GPU in
-    FP64: the test's iT = 572, energies within 1e-10 of the reference's run,
-    velocity and pressure L2 errors within 1e-9 of it and within 1e-6 of the
-    table's 0.056544195 and 0.070186678, fluid populations within 1e-12"""
+@pytest.mark.parametrize("n", [11, 21])
+def test_gpu_poiseuille_nonforced_bouzidi_matches_reference(n):
+    """testPoiseuille3DBGK.cpp:63-64 (non-forced, bouzidi, N = 11 and 21:
+    Bouzidi wall, velocity Bouzidi inflow, InterpolatedPressure outflow) on
+    the GPU in FP64, with the post-processor lists and the inflow's wall
+    coefficients computed by this package from the geometry
+    (setBouzidiBoundary and setBouzidiVelocity restated): the test's
+    iT = 572 and 1400, energies within 1e-10 of the reference's run,
+    velocity, pressure and strain-rate L2 errors within 1e-9 of it and within
+    1e-6 of the table's, fluid populations within 1e-12"""
+    from tests.golden_runs import KAT_TABLE, nonforced_bouzidi_lists
     from tests.test_golden import _pressure_error_l2
 
-    meta, d = load("poiseuille11_nonforced_bouzidi_f64")
+    name = f"poiseuille{n}_nonforced_bouzidi_f64"
+    row = KAT_TABLE[name]
+    meta, d = load(name)
+    (bc, bq), (vc, vq, vv) = nonforced_bouzidi_lists(meta, d)
+    print(f"[gpu] {name}: {bc.size} wall and {vc.size} inflow Bouzidi cells", flush=True)
     lat = build(Lattice, meta, d, "f64")
-    lat.set_bouzidi(d["bz_cells"], d["bz_q"])
-    lat.set_bouzidi_velocity(d["bzv_cells"], d["bzv_q"], d["bzv_v"])
+    lat.set_bouzidi(bc, bq)
+    lat.set_bouzidi_velocity(vc, vq, vv)
     it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
-    assert it == 572
+    assert it == meta["iT"] == row[0]
     np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
     err = velocity_error_l2(lat, meta, d)
-    assert_strain(lat, meta, d, "poiseuille11_nonforced_bouzidi_f64")
+    assert_strain(lat, meta, d, name)
     perr = _pressure_error_l2(lat, meta, d)
     print("nonforced bouzidi gpu velocity", err, "pressure", perr)
     assert abs(err - meta["velocityErrorL2"]) <= 1e-9 and abs(perr - meta["pressureErrorL2"]) <= 1e-9
-    assert abs(err - 0.056544195) <= 1e-6 and abs(perr - 0.070186678) <= 1e-6
+    assert abs(err - row[1]) <= row[4] * 1e-6 and abs(perr - row[2]) <= row[4] * 1e-6
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
 
