@@ -322,7 +322,9 @@ def probe_ceilings(size, storages, orders):
     (include/olbgfx_probe.h): a plain 16-byte copy of the rings, and the 19
     PS rings read only, written only, and read-modified-written in place
     with the step kernel's indexing, block shape and (tuned) block order,
-    after one step of ring shifts"""
+    after one step of ring shifts; ring_inplace_aligned is the in-place
+    pattern with every ring offset zero (each ring's lines aligned with the
+    cell index), the most any alignment of the rings could give"""
     from openlb_amd import probe_bandwidth
 
     out = {}
@@ -330,6 +332,8 @@ def probe_ceilings(size, storages, orders):
         order = orders.get(st, -1)
         out[st] = {pat: round(probe_bandwidth((size, size, size), 3, st, pat, 1, order, 10), 1)
                    for pat in ("copy", "ring_read", "ring_write", "ring_inplace")}
+        out[st]["ring_inplace_aligned"] = round(
+            probe_bandwidth((size, size, size), 3, st, "ring_inplace", 0, order, 10), 1)
         out[st]["block_order"] = order
     return out
 

@@ -129,6 +129,26 @@ typedef struct {
 olbgfx_status olbgfx_lattice_create(const olbgfx_lattice_desc* desc, olbgfx_lattice** out);
 olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* lattice);
 
+/* split steps on (default) or off.  A lattice with velocity-boundary cells
+ * (LocalVelocity, InterpolatedVelocity and their pressure, edge and corner
+ * forms, FixedVelocity) whose most frequent core dynamics is BGK, TRT,
+ * ForcedBGK, ForcedTRT or Smagorinsky BGK and whose other core cells are at
+ * most a quarter steps in two launches: that dynamics' own kernel over every
+ * core cell, skipping the others, and the masked kernel over the list of the
+ * others (the GPU form of the reference's CPU applyDominant,
+ * core/platform/cpu/sisd/operator.h:143-228).  Off: one masked launch.  Both
+ * update every core cell once with the same arithmetic and agree bit for bit
+ * in FP32, BF16 and FP64 storage (in FP16 to the last bit: a store may fuse
+ * its narrowing with an FMA in one kernel and not in the other); steps with
+ * statistics on always take one launch.  No reference counterpart for the
+ * switch itself. */
+olbgfx_status olbgfx_set_split(olbgfx_lattice* lattice, int32_t on);
+
+/* the launch the next step takes: *kind the dominant dynamics kind of a
+ * split step and *residual the number of cells of its list, or -1 and 0 for
+ * one launch.  No reference counterpart. */
+olbgfx_status olbgfx_get_split(olbgfx_lattice* lattice, int32_t* kind, uint32_t* residual);
+
 /* number of HIP devices visible (0 when none); checkPlatform<GPU_HIP>
  * (core/platform/gpu/hip/communicator.hh:45-81) checks the same */
 olbgfx_status olbgfx_device_count(int32_t* count);

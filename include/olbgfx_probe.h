@@ -5,9 +5,12 @@
  * (utilities/timer.hh:72-76).
  *
  * Each probe allocates its own device memory, runs `reps` timed repetitions
- * after one untimed one, frees the memory and returns the bandwidth in GB/s
- * (10^9 bytes per second) of the bytes the pattern moves, timed with HIP
- * events on `stream` (synchronises).
+ * after one untimed one for each launch shape of a small sweep (the copy over
+ * 256..16384 looping workgroups and one workgroup per 256 elements, with
+ * plain and with nontemporal accesses; the ring patterns over the occupancy
+ * caps the step kernel's autotuner tries), frees the memory and returns the best
+ * bandwidth in GB/s (10^9 bytes per second) of the bytes the pattern moves,
+ * timed with HIP events on `stream` (synchronises).
  */
 #ifndef OLBGFX_PROBE_H
 #define OLBGFX_PROBE_H
@@ -21,7 +24,7 @@ extern "C" {
 #endif
 
 /* plain device copy of 19 rings' worth of storage (19 x padded cells x
- * element size), 16-byte loads and stores, grid-stride; moves 2 x that */
+ * element size), 16-byte loads and stores; moves 2 x that */
 #define OLBGFX_PROBE_COPY          0
 /* every core cell loads its 19 populations from the PS rings (the slots of
  * the step kernel) and keeps one sum per block; moves 19 elements per cell */

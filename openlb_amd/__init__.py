@@ -98,6 +98,8 @@ def library() -> ctypes.CDLL:
         "olbgfx_lattice_create": [ctypes.POINTER(_LatticeDesc), ctypes.POINTER(vp)],
         "olbgfx_lattice_destroy": [vp],
         "olbgfx_device_count": [ctypes.POINTER(i32)],
+        "olbgfx_set_split": [vp, i32],
+        "olbgfx_get_split": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint32)],
         "olbgfx_probe_bandwidth": [ctypes.POINTER(_LatticeDesc), i32, u32, i32, u32, vp,
                                    ctypes.POINTER(ctypes.c_double)],
         "olbgfx_set_inflight_limit": [vp, u32],
@@ -363,6 +365,19 @@ class Lattice:
         """threads per block (64..256) and an occupancy cap in waves per SIMD
         (0 = no cap)"""
         _check(library().olbgfx_set_launch(self._h, int(block_threads), int(waves_per_simd)))
+
+    def set_split(self, on: bool = True) -> None:
+        """split steps (the dominant dynamics' own kernel plus a masked
+        launch over the other cells) on or off; results do not change (FP16
+        storage: up to the last bit)"""
+        _check(library().olbgfx_set_split(self._h, 1 if on else 0))
+
+    def get_split(self) -> tuple:
+        """(dominant dynamics kind, residual cells) of the split the next
+        step takes, or (-1, 0) for one launch"""
+        kind, rest = ctypes.c_int32(), ctypes.c_uint32()
+        _check(library().olbgfx_get_split(self._h, ctypes.byref(kind), ctypes.byref(rest)))
+        return kind.value, rest.value
 
     def set_block_order(self, xcd_run_log2: int = -1) -> None:
         """-1: each XCD sweeps one contiguous part; k: XCDs take turns on runs

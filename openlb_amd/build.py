@@ -26,6 +26,12 @@ HIPCC_FLAGS = [
     "-shared",
     # packed FP32 instructions stay off (compiler-generated v_pk_* f32 forms)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b+c is fused only within one source expression, so every kernel that
+    # inlines the same collision rounds it the same way (with the default
+    # "fast" the fusion follows each kernel's own instruction graph); the
+    # split step's dominant kernel and the masked kernel then agree bit for
+    # bit in FP32, BF16 and FP64 storage
+    "-ffp-contract=on",
     "-Wno-unused-result",
     "-Wno-unused-value",
 ]

@@ -122,7 +122,13 @@ struct f16_t {
 template <>
 struct Storage<f16_t> {
   using A = float;
-  // v_cvt_f16_f32 rounds to nearest even and keeps FP16 subnormals
+  // v_cvt_f16_f32 rounds to nearest even and keeps FP16 subnormals.  The
+  // compiler folds these conversions into mixed-precision FMAs (v_fma_mix*)
+  // where the instruction graph allows: exact on loads, and on a store it
+  // rounds a fused FMA once to FP16 where the unfused form rounds to FP32
+  // first.  Kept, because the uniform FP16 kernel is partly issue-bound and
+  // runs 3-4 % slower with the conversions forced apart; so a kernel that
+  // fuses and one that does not can differ in the last bit of FP16 storage.
   static __device__ __forceinline__ float unpack(f16_t h) { return (float)h.v; }
   static __device__ __forceinline__ float load(const f16_t* p) { return (float)p->v; }
   static __device__ __forceinline__ f16_t pack(float v) { return f16_t{(_Float16)v}; }
@@ -167,6 +173,7 @@ struct StepArgs {
   const uint32_t* shift_dev; // device copy of the offsets (graph replay)
   const uint8_t* mask;       // dynamics id per padded cell
   int32_t uniform_id;        // >= 0: every core cell has this id, mask unused
+  int32_t skip_others;       // 1: cells whose id is not uniform_id are left to the residual launch
   int32_t nx, ny, nz, pad;
   uint32_t py, pz;           // padded extent in y and z
   uint32_t nyz;              // ny*nz
@@ -917,6 +924,11 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
   if constexpr (MODE >= 0) {
     id = a.uniform_id;
     kind = MODE;
+    // the dominant launch of a split step: other cells belong to the
+    // residual launch, which also does their ghost writes
+    if (a.skip_others && (int)a.mask[n] != id) {
+      return;
+    }
   } else {
     id = (int)a.mask[n];
   }
@@ -942,9 +954,18 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
       counted = apply_any(f, e, n, a, st_rho, st_usqr);
     }
 
+    // Masked kernels store through slots computed again from an opaque copy
+    // of n, so the 19 load offsets need not stay live across the collision:
+    // 65 VGPRs against 103, up to 8 waves per SIMD instead of 4, +17 % on the
+    // FP16 porous channel.  The uniform kernels keep their offsets (no gain
+    // measured there).
+    uint32_t m = n;
+    if constexpr (MODE < 0) {
+      asm("" : "+v"(m));
+    }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      *ring_at<S>(a.pop, a.stride, i, ring_slot(n, sh[i], a.count)) = St::pack(f[i]);
+      *ring_at<S>(a.pop, a.stride, i, ring_slot(m, sh[i], a.count)) = St::pack(f[i]);
     }
   }
 
@@ -1367,6 +1388,28 @@ __global__ void __launch_bounds__(kMaxBlock) step_kernel(const StepArgs<typename
   }
 }
 
+// The residual launch of a split step: the core cells (linear core indices,
+// ascending) whose dynamics is not the dominant one, with the masked code
+// (MODE -1, or -2 when boundary kinds are present).  Together with the
+// dominant launch it updates every core cell once, as the reference's CPU
+// applyDominant does with its mask (cpu/sisd/operator.h:143-228).
+template <typename S, int MODE>
+__global__ void __launch_bounds__(256) residual_kernel(const StepArgs<typename Storage<S>::A> a,
+                                                       const uint32_t* cells, uint32_t n)
+{
+  using A = typename Storage<S>::A;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) {
+    return;
+  }
+  uint32_t sh[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) sh[i] = a.shift[i];
+  A st_rho = A(0), st_usqr = A(0);
+  bool counted = false;
+  step_cell<S, MODE>(a, sh, cells[k], st_rho, st_usqr, counted);
+}
+
 // ---------------------------------------------------------------------------
 // the ensemble kernel: one persistent launch advances one group of member
 // lattices by one step.  The work is a list of tiles (member, first core
@@ -1712,6 +1755,13 @@ struct olbgfx_lattice {
   uint32_t inflight_limit = 0;
   std::vector<hipEvent_t> inflight_events;
   uint64_t inflight_count = 0;
+  // split step: the dominant dynamics runs its own kernel over all core
+  // cells (skipping the others), the residual cells a masked kernel over a list
+  int32_t split_id = -1;             // dominant table slot, -1: no split
+  uint32_t* residual_dev = nullptr;  // linear core indices of the other cells
+  uint32_t residual_n = 0;
+  uint64_t split_version = ~0ull;
+  bool split_enabled = true;         // olbgfx_set_split
 };
 
 namespace {
@@ -1754,6 +1804,7 @@ olbgfx_status fill_args(olbgfx_lattice* L, StepArgs<A>& a, bool device_shifts)
   a.shift_dev = device_shifts ? L->shift_dev : nullptr;
   a.mask = L->mask;
   a.uniform_id = L->uniform_id;
+  a.skip_others = 0;
   a.nx = D.nx;
   a.ny = D.ny;
   a.nz = D.nz;
@@ -1859,6 +1910,74 @@ olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a
   return OLBGFX_OK;
 }
 
+// the dominant dynamics and the residual list of a split step, recomputed
+// when dynamics change.  Split when the lattice holds velocity-boundary kinds
+// (whose masked kernel carries their code and runs at 3 waves per SIMD), the
+// most frequent core dynamics has a kernel of its own (BGK, TRT, their
+// forced forms, Smagorinsky BGK) and the other core cells are at most a
+// quarter.  Without boundary kinds the masked kernel is nearly as fast as the
+// dominant one, and the residual launch loses more than the split gains:
+// a wall cell at the end of a z row touches 19 cache lines of its own there,
+// which one launch shares with the row's other cells.
+olbgfx_status prepare_split(olbgfx_lattice* L)
+{
+  if (L->split_version == L->version) {
+    return OLBGFX_OK;
+  }
+  L->split_version = L->version;
+  L->split_id = -1;
+  if (L->residual_dev) {
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipFree(L->residual_dev));
+    L->residual_dev = nullptr;
+  }
+  L->residual_n = 0;
+  if (!L->split_enabled || L->uniform_id >= 0 || !L->boundary_kinds) {
+    return OLBGFX_OK;
+  }
+  const olbgfx_lattice_desc& D = L->desc;
+  uint64_t count[OLBGFX_MAX_DYNAMICS] = {};
+  for (int x = 0; x < D.nx; ++x) {
+    for (int y = 0; y < D.ny; ++y) {
+      const size_t base = ((size_t)(x + D.padding) * L->py + (y + D.padding)) * L->pz + D.padding;
+      for (int z = 0; z < D.nz; ++z) {
+        ++count[L->host_mask[base + z]];
+      }
+    }
+  }
+  int dom = 0;
+  for (int k = 1; k < OLBGFX_MAX_DYNAMICS; ++k) {
+    dom = count[k] > count[dom] ? k : dom;
+  }
+  const int kind = L->table[dom].kind;
+  const bool own_kernel = kind == OLBGFX_BGK || kind == OLBGFX_TRT || kind == OLBGFX_FORCED_BGK ||
+                          kind == OLBGFX_FORCED_TRT || kind == OLBGFX_SMAGORINSKY_BGK;
+  const uint64_t ncore = (uint64_t)D.nx * D.ny * D.nz;
+  if (!own_kernel || 4 * (ncore - count[dom]) > ncore) {
+    return OLBGFX_OK;
+  }
+  std::vector<uint32_t> rest;
+  rest.reserve(ncore - count[dom]);
+  uint32_t lin = 0;
+  for (int x = 0; x < D.nx; ++x) {
+    for (int y = 0; y < D.ny; ++y) {
+      const size_t base = ((size_t)(x + D.padding) * L->py + (y + D.padding)) * L->pz + D.padding;
+      for (int z = 0; z < D.nz; ++z, ++lin) {
+        if (L->host_mask[base + z] != dom) {
+          rest.push_back(lin);
+        }
+      }
+    }
+  }
+  if (!rest.empty()) {
+    HIP_TRY(hipMalloc(&L->residual_dev, sizeof(uint32_t) * rest.size()));
+    HIP_TRY(hipMemcpy(L->residual_dev, rest.data(), sizeof(uint32_t) * rest.size(), hipMemcpyHostToDevice));
+  }
+  L->residual_n = (uint32_t)rest.size();
+  L->split_id = dom;
+  return OLBGFX_OK;
+}
+
 template <typename S>
 olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream)
 {
@@ -1886,13 +2005,31 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
   if (L->boundary_kinds) {
     mode = kMaskedBoundary;
   }
+  const bool split = !L->stats_on && prepare_split(L) == OLBGFX_OK && L->split_id >= 0;
+  StepArgs<A> rest = a;
+  if (split) {
+    // dominant kind over every core cell, the others from the list
+    a.uniform_id = L->split_id;
+    a.skip_others = 1;
+    rest.uniform_id = -1;
+  }
+  const int dominant = split ? L->table[L->split_id].kind : mode;
   for (uint32_t k = 0; k < steps; ++k) {
     for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
     if (L->stats_on) {
       launch_mode<S, true>(mode, grid, block, L->lds_reserve, stream, a);
       stats_finalize<<<1, 256, 0, stream>>>(L->stats_partial, a.nblocks, L->stats_out);
     } else {
-      launch_mode<S, false>(mode, grid, block, L->lds_reserve, stream, a);
+      launch_mode<S, false>(dominant, grid, block, L->lds_reserve, stream, a);
+      if (split && L->residual_n > 0) {
+        for (int i = 0; i < Q; ++i) rest.shift[i] = L->shift[i];
+        const dim3 rgrid((L->residual_n + 255) / 256);
+        if (L->boundary_kinds) {
+          residual_kernel<S, kMaskedBoundary><<<rgrid, 256, 0, stream>>>(rest, L->residual_dev, L->residual_n);
+        } else {
+          residual_kernel<S, -1><<<rgrid, 256, 0, stream>>>(rest, L->residual_dev, L->residual_n);
+        }
+      }
     }
     HIP_TRY(hipGetLastError());
     // stream(): s_i -= d_i (mod count), core/blockLattice.hh:316-338
@@ -2339,6 +2476,7 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* L)
   hipFree(L->bzv_v);
   hipFree(L->stats_partial);
   hipFree(L->stats_out);
+  hipFree(L->residual_dev);
   for (hipEvent_t e : L->inflight_events) {
     hipEventDestroy(e);
   }
@@ -2364,6 +2502,36 @@ olbgfx_status olbgfx_set_inflight_limit(olbgfx_lattice* L, uint32_t steps)
   }
   L->inflight_limit = steps;
   L->inflight_count = 0;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_set_split(olbgfx_lattice* L, int32_t on)
+{
+  if (check(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  L->split_enabled = on != 0;
+  L->split_version = ~0ull;
+  return OLBGFX_OK;
+}
+
+olbgfx_status olbgfx_get_split(olbgfx_lattice* L, int32_t* kind, uint32_t* residual)
+{
+  if (check(L) != OLBGFX_OK || !kind || !residual) {
+    return OLBGFX_ERR_INVALID;
+  }
+  *kind = -1;
+  *residual = 0;
+  if (prepare_boundary(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_INVALID;
+  }
+  if (prepare_split(L) != OLBGFX_OK) {
+    return OLBGFX_ERR_HIP;
+  }
+  if (!L->stats_on && L->split_id >= 0) {
+    *kind = L->table[L->split_id].kind;
+    *residual = L->residual_n;
+  }
   return OLBGFX_OK;
 }
 

@@ -30,10 +30,12 @@ def _fixture(name):
 # amplifies rounding: OpenLB's own CPU_SIMD run lies simd_vs_sisd_u /
 # simd_vs_sisd_p from its CPU_SISD run (fixture), and the GPU's FP32 run is
 # held to three times that spread (at least 1e-5).  FP16 stores the shifted
-# populations with 11 significant bits; the velocity keeps ~1e-3 of the lid
-# speed, the pressure, (rho - 1) / 3 with rho - 1 ~ 1e-4 here, only ~1e-2 of
-# its range
-TOLERANCE = {"f16": (2e-3, 5e-2)}
+# populations with 11 significant bits: its velocity is held to the relative
+# L2 error of 1e-3 that FP16 storage is specified to keep, its single worst
+# cell (which near tau = 1/2 depends on the order of the roundings) to 1e-2
+# of the lid speed, and the pressure, (rho - 1) / 3 with rho - 1 ~ 1e-4 here,
+# to 5e-2 of its range
+TOLERANCE = {"f16": {"l2_u": 1e-3, "max_u": 1e-2, "max_p": 5e-2}}
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
@@ -60,14 +62,16 @@ def test_openlb_case_matches_cpu_platform(fixture, storage, tmp_path):
     p = fields["physPressure"]["physPressure"].astype(np.float64)
     p_ref = ref["physPressure/physPressure"].astype(np.float64)
     err_p = np.abs(p - p_ref).max() / np.abs(p_ref).max()
-    print(f"max |du|/max|u| {err_u:.3e}, max |dp|/max|p| {err_p:.3e}", flush=True)
+    l2_u = float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
+    print(f"max |du|/max|u| {err_u:.3e}, max |dp|/max|p| {err_p:.3e}, L2 |du|/|u| {l2_u:.3e}", flush=True)
     if storage == "f32":
         tol_u = max(1e-5, 3 * meta["simd_vs_sisd_u"])
         tol_p = max(1e-5, 3 * meta["simd_vs_sisd_p"])
         print(f"OpenLB CPU_SIMD vs CPU_SISD: {meta['simd_vs_sisd_u']:.3e}, {meta['simd_vs_sisd_p']:.3e}", flush=True)
+        assert err_u <= tol_u and err_p <= tol_p
     else:
-        tol_u, tol_p = TOLERANCE[storage]
-    assert err_u <= tol_u and err_p <= tol_p
+        tol = TOLERANCE[storage]
+        assert l2_u <= tol["l2_u"] and err_u <= tol["max_u"] and err_p <= tol["max_p"]
 
 
 def test_openlb_tgv3d_dns_matches_cpu_platform(tmp_path):
