@@ -1104,48 +1104,100 @@ __device__ void one_sided(const StepArgs<typename Storage<S>::A>& a, uint32_t n,
   }
 }
 
+// The cells whose momenta the edge or corner processor of cell n reads,
+// slot by slot.  Edge, plane p (the zero component of the normal), d1 and d2
+// the other axes with normal components n1 and n2: 0: n - n1 e_d1,
+// 1: n - n2 e_d2, 2: n - 2 n1 e_d1, 3: n - 2 n2 e_d2, 4: n + e_p, 5: n - e_p,
+// 6: n.  Corner, normal N: 0-2: n - N_k e_k, 3-5: n - 2 N_k e_k, 6: n.
+constexpr int kPostSlots = 7;
+
+__device__ __forceinline__ uint32_t post_neighbor(uint32_t n, const int (&N)[3], const int64_t (&step)[3], int slot)
+{
+  if (slot == 6) {
+    return n;
+  }
+  if (N[0] == 0 || N[1] == 0 || N[2] == 0) {
+    const int p = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
+    const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
+    switch (slot) {
+    case 0: return (uint32_t)((int64_t)n - N[d1] * step[d1]);
+    case 1: return (uint32_t)((int64_t)n - N[d2] * step[d2]);
+    case 2: return (uint32_t)((int64_t)n - 2 * N[d1] * step[d1]);
+    case 3: return (uint32_t)((int64_t)n - 2 * N[d2] * step[d2]);
+    case 4: return (uint32_t)((int64_t)n + step[p]);
+    default: return (uint32_t)((int64_t)n - step[p]);
+    }
+  }
+  const int k = slot % 3;
+  const int64_t m = slot < 3 ? 1 : 2;
+  return (uint32_t)((int64_t)n - m * N[k] * step[k]);
+}
+
+// One launch applies the processors of one list (edges, or corners: a
+// corner reads edge cells the edge processors rewrite, so the corners run in
+// a launch of their own after the edges).  A block takes 32 cells; lane
+// `slot` of a cell evaluates the momenta of one of its 7 neighbour cells
+// into LDS, so the ~13 neighbour evaluations of a cell run side by side
+// instead of one after another; lane 0 then combines them and writes the
+// cell's 19 populations, with the arithmetic of
+// OuterVelocityEdgeProcessor3D / OuterVelocityCornerProcessor3D
+// (boundary/boundaryPostProcessors3D.hh:164-294).
+constexpr uint32_t kPostCellsPerBlock = 32;
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_velocity_post(const StepArgs<typename Storage<S>::A> a,
                                                        const uint32_t* cells, uint32_t ncells)
 {
   using A = typename Storage<S>::A;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncells) {
-    return;
-  }
-  const uint32_t n = cells[t];
-  const uint32_t code = a.normal[n];
+  __shared__ A mom[kPostCellsPerBlock][8][4];
+  const uint32_t local = threadIdx.x >> 3;
+  const int slot = (int)(threadIdx.x & 7u);
+  const uint32_t t = blockIdx.x * kPostCellsPerBlock + local;
+  const bool active = t < ncells;
+  const uint32_t n = active ? cells[t] : 0u;
+  const uint32_t code = active ? a.normal[n] : 13u;
   const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
   const int64_t step[3] = {(int64_t)a.py * a.pz, (int64_t)a.pz, 1};
+  if (active && slot < kPostSlots) {
+    const uint32_t m = post_neighbor(n, N, step, slot);
+    A u[3];
+    momenta_u<S>(a, m, u);
+    mom[local][slot][0] = momenta_rho<S>(a, m);
+    mom[local][slot][1] = u[0];
+    mom[local][slot][2] = u[1];
+    mom[local][slot][3] = u[2];
+  }
+  __syncthreads();
+  if (!active || slot != 0) {
+    return;
+  }
   const DynEntry<A>& e = a.table[a.mask[n]];
+  // DirectedGradients3D along a normal (finiteDifference3D.h:132-155):
+  // -orient * (-3 u0 + 4 u1 - u2) / 2 with u1, u2 one and two cells inwards
+  auto one_sided = [&](int s1, int s2, int orient, A (&d)[3]) {
+    for (int b = 0; b < 3; ++b) {
+      d[b] = A(-orient) * ((-A(3) * mom[local][6][1 + b] + A(4) * mom[local][s1][1 + b] -
+                            A(1) * mom[local][s2][1 + b]) / A(2));
+    }
+  };
   A rho;
   A du[3][3];
   if (N[0] == 0 || N[1] == 0 || N[2] == 0) {
     // OuterVelocityEdgeProcessor3D<plane, normal1, normal2>
     const int p = N[0] == 0 ? 0 : (N[1] == 0 ? 1 : 2);
     const int d1 = (p + 1) % 3, d2 = (p + 2) % 3;
-    const int n1 = N[d1], n2 = N[d2];
-    const A rho10 = momenta_rho<S>(a, (uint32_t)((int64_t)n - n1 * step[d1]));
-    const A rho01 = momenta_rho<S>(a, (uint32_t)((int64_t)n - n2 * step[d2]));
-    const A rho20 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * n1 * step[d1]));
-    const A rho02 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * n2 * step[d2]));
+    const A rho10 = mom[local][0][0], rho01 = mom[local][1][0];
+    const A rho20 = mom[local][2][0], rho02 = mom[local][3][0];
     rho = A(2) / A(3) * (rho01 + rho10) - A(1) / A(6) * (rho02 + rho20);
-    A up[3], um[3];
-    momenta_u<S>(a, (uint32_t)((int64_t)n + step[p]), up);
-    momenta_u<S>(a, (uint32_t)((int64_t)n - step[p]), um);
-    for (int b = 0; b < 3; ++b) du[p][b] = (up[b] - um[b]) / A(2);
-    one_sided<S>(a, n, step[d1], n1, du[d1]);
-    one_sided<S>(a, n, step[d2], n2, du[d2]);
+    for (int b = 0; b < 3; ++b) du[p][b] = (mom[local][4][1 + b] - mom[local][5][1 + b]) / A(2);
+    one_sided(0, 2, N[d1], du[d1]);
+    one_sided(1, 3, N[d2], du[d2]);
   } else {
     // OuterVelocityCornerProcessor3D<xNormal, yNormal, zNormal>
-    const A rho100 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[0] * step[0]));
-    const A rho010 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[1] * step[1]));
-    const A rho001 = momenta_rho<S>(a, (uint32_t)((int64_t)n - N[2] * step[2]));
-    const A rho200 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[0] * step[0]));
-    const A rho020 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[1] * step[1]));
-    const A rho002 = momenta_rho<S>(a, (uint32_t)((int64_t)n - 2 * N[2] * step[2]));
+    const A rho100 = mom[local][0][0], rho010 = mom[local][1][0], rho001 = mom[local][2][0];
+    const A rho200 = mom[local][3][0], rho020 = mom[local][4][0], rho002 = mom[local][5][0];
     rho = A(4) / A(9) * (rho001 + rho010 + rho100) - A(1) / A(9) * (rho002 + rho020 + rho200);
-    for (int ax = 0; ax < 3; ++ax) one_sided<S>(a, n, step[ax], N[ax], du[ax]);
+    for (int ax = 0; ax < 3; ++ax) one_sided(ax, 3 + ax, N[ax], du[ax]);
   }
   // regularised populations (boundaryPostProcessors3D.hh:196-215)
   const A s_to_pi = -rho / A(3) / e.omega;
@@ -1893,12 +1945,13 @@ olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a
     HIP_TRY(hipGetLastError());
   }
   if (L->post_edges) {
-    k_velocity_post<S><<<(L->post_edges + 255) / 256, 256, 0, stream>>>(a, L->post_dev, L->post_edges);
+    k_velocity_post<S><<<(L->post_edges + kPostCellsPerBlock - 1) / kPostCellsPerBlock, 256, 0, stream>>>(
+        a, L->post_dev, L->post_edges);
     HIP_TRY(hipGetLastError());
   }
   if (L->post_corners) {
-    k_velocity_post<S><<<(L->post_corners + 255) / 256, 256, 0, stream>>>(a, L->post_dev + L->post_edges,
-                                                                         L->post_corners);
+    k_velocity_post<S><<<(L->post_corners + kPostCellsPerBlock - 1) / kPostCellsPerBlock, 256, 0, stream>>>(
+        a, L->post_dev + L->post_edges, L->post_corners);
     HIP_TRY(hipGetLastError());
   }
   if (L->fd_n) {
