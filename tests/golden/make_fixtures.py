@@ -166,7 +166,7 @@ def cavity_reduced(binary: str, tag: str, res: int = 64, steps: int = 1000, mode
     print("wrote", name, sums)
 
 
-ENSEMBLE_STEPS = 1000
+ENSEMBLE_STEPS = 2000
 
 
 def ensemble_member_picks(members) -> dict:
@@ -179,7 +179,7 @@ def ensemble_member_picks(members) -> dict:
 
 def ensemble_members() -> None:
     """three members of the 64-cavity ensemble (cases.ensemble_cavities, seed
-    20261202) through the reference's FP32 path, 1000 steps from rest: a
+    20261202) through the reference's FP32 path, ENSEMBLE_STEPS (2000) steps from rest: a
     member of edge n is the cavity3dBenchmark geometry at RESOLUTION n - 1
     (n^3 core cells) with its own lid speed U; the reference forms
     nu = U RES / Re, so Re' = Re (n - 1) / n gives the member's nu = U n / Re"""
@@ -622,6 +622,27 @@ def poiseuille_bouzidi_nonforced(n: int = 11) -> None:
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print("wrote", name, meta, {k: v.shape for k, v in lists.items()})
 
+
+def poiseuille_bouzidi_nonforced_light(n: int = 41) -> None:
+    """testPoiseuille3DBGK's non-forced bouzidi row at N = 41 (FP64, one
+    thread, run to convergence), kept small: the geometry (materials and the
+    discrete normals of the outflow's InterpolatedPressure), the energy
+    series and the converged errors.  Its start, Bouzidi lists and analytic
+    solutions are restated from the geometry (tests/golden_runs.py
+    build_nonforced_bouzidi), pinned at N = 11 and 21 against full fixtures"""
+    name = f"poiseuille{n}_nonforced_bouzidi_light_f64"
+    path = os.path.join(RUN, "poiseuille_bouzidi_nonforced.bin")
+    run("olb_ref_d", ["poiseuille_bouzidi_nonforced", str(n), path], threads=1)
+    meta, arrays = read_dump(path)
+    out = {
+        "meta": np.array(json.dumps(meta)),
+        "materials": arrays["materials"].astype(np.uint8),
+        "normal_type": np.asarray(arrays["normal_type"]).astype(np.int8),
+        "normals": np.asarray(arrays["normals"]).astype(np.int8),
+        "energy": arrays["energy"],
+    }
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, meta)
 
 if __name__ == "__main__":
     main()

@@ -296,6 +296,7 @@ KAT_TABLE = {
     "poiseuille11_nonforced_interpolated_f64": (570, 0.04976679, 0.10483063, 0.04959827, 1),
     "poiseuille11_nonforced_bouzidi_f64": (572, 0.056544195, 0.070186678, 0.088491618, 1),
     "poiseuille21_nonforced_bouzidi_f64": (1400, 0.01624583, 0.024478267, 0.046465458, 1),
+    "poiseuille41_nonforced_bouzidi_light_f64": (4077, 0.00445271, 0.00583554, 0.02117142, 2),
 }
 
 
@@ -473,3 +474,79 @@ def step_cavity_oracle(storage: str):
                 o.collide_and_stream(1)
         runs.append(np.asarray(o.get_populations()[:, cells], dtype=np.float64))
     return c, d, cells, runs[0], runs[1] == runs[2]
+
+
+def nonforced_poiseuille_state(meta, materials) -> dict:
+    """The t = 0 state and the analytic solutions of the non-forced bouzidi
+    rows of testPoiseuille3DBGK.cpp, restated from the geometry and the unit
+    converter (lx = 2, r = 1/2, Re = 10, charPhysVelocity 1, physViscosity
+    1/Re):
+      - prepareLattice (:218-241): rho = AnalyticalLinear3D(-p0/lx * invCs2,
+        0, 0, p0 * invCs2 + 1) with p0 the lattice pressure of 4 nu u lx / r^2,
+        u = CirclePoiseuille3D((lx, r, r), x, charLatticeVelocity, r) on
+        materials 1-4 and 0 on material 0; iniEquilibrium sets the shifted
+        second-order equilibrium on the BGK-based cells (materials 1 and 4)
+        and leaves the NoDynamics cells (0, 2, 3: the outside, the Bouzidi
+        wall and inflow) at f_bar = 0, whose equilibrium::None returns 0;
+        the RHO and VELOCITY fields carry rho and u (defineRhoU);
+      - error (:253-298) on the fluid cells: the physical Poiseuille velocity
+        (maxVelocity 1), the pressure 4 nu u lx / r^2 (1 - x / lx) and
+        CirclePoiseuilleStrainRate3D (functors/analytical/frameChangeF3D.hh:
+        406-436).
+    BlockLattice::initialize then applies the PostStream processors once
+    (core/blockLattice.hh:56-67); the caller does that with post_process().
+    Returns f0 (19 x cells, shifted), rho, u, fluid, usol_fluid, psol_fluid,
+    ssol_fluid."""
+    from openlb_amd import bouzidi
+    from tests.helpers import C, T
+
+    p = meta["padding"]
+    shape = padded_shape(meta)
+    n = int(np.prod(shape))
+    mats = np.asarray(materials).reshape(-1)
+    origin, dx = np.asarray(origin_of(meta)), meta["delta"]
+    xyz = np.array(np.unravel_index(np.arange(n), shape)).T - p
+    phys = origin[None, :] + xyz * dx
+    lx, r, nu = 2.0, 0.5, 0.1
+    u = bouzidi.CirclePoiseuille3D([lx, r, r], [1.0, 0.0, 0.0], 1.0 / meta["u_factor"], r)(phys)
+    u[mats == 0] = 0.0
+    p0 = 4.0 * nu * 1.0 * lx / (r * r) / meta["p_factor"]
+    rho = (-p0 / lx * 3.0) * phys[:, 0] + (p0 * 3.0 + 1.0)
+    c = np.asarray(C, dtype=np.float64)
+    cu = u @ c.T
+    usq = np.sum(u * u, axis=1)
+    f0 = (rho[:, None] * T[None, :] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq[:, None]) - T[None, :]).T
+    f0[:, ~np.isin(mats, (1, 4))] = 0.0
+    fluid = np.flatnonzero(mats == 1)
+    pf = phys[fluid]
+    usol = bouzidi.CirclePoiseuille3D([lx, r, r], [1.0, 0.0, 0.0], 1.0, r)(pf)
+    p0phys = 4.0 * nu * 1.0 * lx / (r * r)
+    psol = (-p0phys / lx) * pf[:, 0] + p0phys
+    ssol = np.zeros((fluid.size, 9))
+    dudy = 1.0 * (-2.0 * (pf[:, 1] - r) / (r * r))
+    dudz = 1.0 * (-2.0 * (pf[:, 2] - r) / (r * r))
+    ssol[:, 1] = ssol[:, 3] = dudy / 2.0
+    ssol[:, 2] = ssol[:, 6] = dudz / 2.0
+    return {"f0": f0, "rho": rho, "u": u, "fluid": fluid, "usol_fluid": usol, "psol_fluid": psol,
+            "ssol_fluid": ssol}
+
+
+def build_nonforced_bouzidi(lattice_cls, meta, d, storage: str):
+    """a non-forced bouzidi row set up from the geometry alone: the
+    materials and discrete normals of `d`, the t = 0 state of
+    nonforced_poiseuille_state, the Bouzidi lists of nonforced_bouzidi_lists,
+    then the PostStream processors once, as BlockLattice::initialize does.
+    Returns the lattice and the analytic solutions (a dict shaped like a
+    fixture for the error norms)"""
+    st = nonforced_poiseuille_state(meta, d["materials"])
+    n = st["f0"].shape[1]
+    start = {"materials": d["materials"], "normal_type": d["normal_type"], "normals": d["normals"],
+             "cells": np.arange(n, dtype=np.int64), "f0": st["f0"], "u0": st["u"], "rho0": st["rho"]}
+    lat = build(lattice_cls, meta, start, storage)
+    (bc, bq), (vc, vq, vv) = nonforced_bouzidi_lists(meta, d)
+    lat.set_bouzidi(bc, bq)
+    lat.set_bouzidi_velocity(vc, vq, vv)
+    lat.post_process()
+    sol = {"materials": d["materials"], "fluid": st["fluid"], "usol_fluid": st["usol_fluid"],
+           "psol_fluid": st["psol_fluid"], "ssol_fluid": st["ssol_fluid"]}
+    return lat, sol

@@ -203,3 +203,58 @@ def test_oracle_poiseuille21_nonforced_bouzidi_known_answer():
     assert abs(err - 0.01624583) <= 1e-6 and abs(perr - 0.024478267) <= 1e-6
     sel = core_fluid(meta, d)
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["poiseuille11_nonforced_bouzidi_f64", "poiseuille21_nonforced_bouzidi_f64"])
+def test_restated_poiseuille_start_matches_reference(name):
+    """the restated t = 0 state of the non-forced rows (prepareLattice's
+    rho and u, equilibrium on the BGK-based cells, NoDynamics cells at rest)
+    and the analytic solutions of error() against the reference's own:
+    fluid, inflow and outflow populations after initialize()'s PostStream
+    pass within 1e-15 (the Bouzidi wall's own cells differ where the
+    concurrent list resolves mutual q = 1/2 pairs, which never reach the
+    fluid), velocity, pressure and strain-rate solutions within 5e-16"""
+    from tests.golden_runs import nonforced_poiseuille_state
+
+    meta, d = load(name)
+    st = nonforced_poiseuille_state(meta, d["materials"])
+    np.testing.assert_array_equal(st["fluid"], d["fluid"])
+    for k in ("usol_fluid", "psol_fluid", "ssol_fluid"):
+        assert np.max(np.abs(st[k] - d[k])) <= 5e-16, k
+    lat = build(OracleLattice, meta, {"materials": d["materials"], "normal_type": d["normal_type"],
+                                      "normals": d["normals"], "cells": np.arange(st["f0"].shape[1]),
+                                      "f0": st["f0"], "u0": st["u"], "rho0": st["rho"]}, "f64")
+    (bc, bq), (vc, vq, vv) = nonforced_bouzidi_lists(meta, d)
+    lat.set_bouzidi(bc, bq)
+    lat.set_bouzidi_velocity(vc, vq, vv)
+    lat.post_process()
+    f = lat.get_populations()[:, d["cells"]]
+    mats = d["materials"][d["cells"]]
+    on = np.isin(mats, (1, 3, 4))
+    assert np.max(np.abs(f[:, on] - d["f0"][:, on])) <= 1e-15
+
+
+@pytest.mark.parametrize("name", ["poiseuille11_nonforced_bouzidi_f64", "poiseuille21_nonforced_bouzidi_f64",
+                                  "poiseuille41_nonforced_bouzidi_light_f64"])
+def test_oracle_poiseuille_nonforced_bouzidi_from_geometry(name):
+    """testPoiseuille3DBGK.cpp:63-65, the non-forced bouzidi rows at N = 11,
+    21 and 41, set up from the geometry alone (restated start, Bouzidi lists
+    and analytic solutions): the table's iT (572, 1400, 4077), energies
+    within 1e-10 of the reference built here, velocity, pressure and
+    strain-rate L2 errors within 1e-9 of it and within errorFactor x 1e-6 of
+    the table (errorFactor 2 at N = 41)"""
+    from tests.golden_runs import KAT_TABLE, build_nonforced_bouzidi, strain_error_l2
+    from tests.test_golden import _pressure_error_l2
+
+    meta, d = load(name)
+    row = KAT_TABLE[name]
+    lat, sol = build_nonforced_bouzidi(OracleLattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == meta["iT"] == row[0]
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    errs = (velocity_error_l2(lat, meta, sol), _pressure_error_l2(lat, meta, sol), strain_error_l2(lat, meta, sol))
+    print(name, "errors", errs)
+    for got, ref, table in zip(errs, (meta["velocityErrorL2"], meta["pressureErrorL2"], meta["strainErrorL2"]),
+                               (row[1], row[2], row[3])):
+        assert abs(got - ref) <= 1e-9
+        assert abs(got - table) <= row[4] * 1e-6

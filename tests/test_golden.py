@@ -362,7 +362,7 @@ def test_oracle_poiseuille_nonforced_bounceback_known_answer():
 def test_oracle_ensemble_smallest_member_matches_reference():
     """the smallest cavity of the 64-member ensemble (cases.ensemble_cavities,
     its own edge and lid speed, Re = 100) against the reference's FP32 run of
-    the same cavity, 1000 steps: this pins the fixture's mapping (edge n =
+    the same cavity, 2000 steps: this pins the fixture's mapping (edge n =
     RESOLUTION n - 1, Re' = Re (n - 1) / n) that the GPU ensemble test uses"""
     from tests.golden.make_fixtures import ENSEMBLE_STEPS, ensemble_member_picks
     from tests.golden_runs import cavity_reduced

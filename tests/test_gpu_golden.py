@@ -453,6 +453,33 @@ def test_gpu_poiseuille_nonforced_bouzidi_matches_reference(n):
     assert rel_err_unshifted(lat.get_populations()[:, d["fluid"][sel]], d["f1_fluid"][:, sel]) <= 1e-12
 
 
+@pytest.mark.timeout(300)
+def test_gpu_poiseuille41_nonforced_bouzidi_from_geometry():
+    """testPoiseuille3DBGK.cpp:65 (non-forced, bouzidi, N = 41, errorFactor 2)
+    on the GPU in FP64, set up from the geometry alone (the restated t = 0
+    state, Bouzidi lists and analytic solutions of tests/golden_runs.py,
+    pinned at N = 11 and 21 against full fixtures): iT = 4077, energies
+    within 1e-10 of the reference built here, the three L2 errors within
+    1e-9 of it and within 2e-6 of the table"""
+    from tests.golden_runs import KAT_TABLE, build_nonforced_bouzidi, strain_error_l2
+    from tests.test_golden import _pressure_error_l2
+
+    name = "poiseuille41_nonforced_bouzidi_light_f64"
+    row = KAT_TABLE[name]
+    meta, d = load(name)
+    print(f"[gpu] {name}: building from the geometry", flush=True)
+    lat, sol = build_nonforced_bouzidi(Lattice, meta, d, "f64")
+    it, energy = value_tracer_run(lat, meta["deltaT"], meta["residuum"], meta["max_steps"])
+    assert it == meta["iT"] == row[0]
+    np.testing.assert_allclose(energy, d["energy"], rtol=1e-10)
+    errs = (velocity_error_l2(lat, meta, sol), _pressure_error_l2(lat, meta, sol), strain_error_l2(lat, meta, sol))
+    print("N = 41 gpu errors", errs)
+    for got, ref, table in zip(errs, (meta["velocityErrorL2"], meta["pressureErrorL2"], meta["strainErrorL2"]),
+                               (row[1], row[2], row[3])):
+        assert abs(got - ref) <= 1e-9
+        assert abs(got - table) <= row[4] * 1e-6
+
+
 def test_gpu_poiseuille_nonforced_bounceback_known_answer():
     """testPoiseuille3DBGK.cpp:61 (non-forced, bounceBack, N = 11) on the GPU
     in FP64: BounceBack wall, InterpolatedVelocity inflow, InterpolatedPressure
