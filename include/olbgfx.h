@@ -134,8 +134,9 @@ olbgfx_status olbgfx_lattice_destroy(olbgfx_lattice* lattice);
  * forms, FixedVelocity) whose most frequent core dynamics is BGK, TRT,
  * ForcedBGK, ForcedTRT or Smagorinsky BGK and whose other core cells are at
  * most a quarter steps in two launches: that dynamics' own kernel over every
- * core cell, skipping the others, and the masked kernel over the list of the
- * others (the GPU form of the reference's CPU applyDominant,
+ * core cell (with 4- and 8-byte storage also updating the LocalVelocity and
+ * FixedVelocity cells), skipping the others, and the masked kernel over the
+ * list of the others (the GPU form of the reference's CPU applyDominant,
  * core/platform/cpu/sisd/operator.h:143-228).  Off: one masked launch.  Both
  * update every core cell once with the same arithmetic and agree bit for bit
  * in FP32, BF16 and FP64 storage (in FP16 to the last bit: a store may fuse

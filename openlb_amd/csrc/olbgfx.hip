@@ -47,6 +47,12 @@ constexpr int Q = 19;
 // registers per lane below what the D3Q19 collision needs without spilling
 constexpr uint32_t kMaxBlock = 256;
 constexpr int kMaskedBoundary = -2;  // step-kernel MODE: masked, LocalVelocity kinds compiled in
+// step-kernel MODE kWithFaces + K: the dominant launch of a split step whose
+// dominant kind is K, with the LocalVelocity and FixedVelocity cells (the
+// faces of a velocity boundary and its outer edges and corners, before their
+// post-processors) updated inline; every other cell is left to the residual
+// launch
+constexpr int kWithFaces = 32;
 
 // the boundary kinds that read the cell's discrete normal and run in the
 // masked kernel with boundary kinds compiled in
@@ -770,6 +776,32 @@ __device__ __forceinline__ void flat_normal(uint32_t code, int& dir, int& orient
 
 // the boundary kinds that read the discrete normal (the masked kernel with
 // boundary kinds); returns true: all yield a statistic
+// a flat LocalVelocity face (CombinedRLBdynamics with
+// RegularizedVelocityBoundaryTuple): the regularised BGK of local_velocity
+template <typename A>
+__device__ __forceinline__ void face_local_velocity(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
+                                                    const StepArgs<A>& a, const A (&u)[3], A& rho)
+{
+  int dir, orient;
+  flat_normal(a.normal[n], dir, orient);
+  local_velocity(f, dir, orient, u, e.omega, rho);
+}
+
+// BulkDensity + FixedVelocityMomentumGeneric, BGK (collision.h:57-80)
+template <typename A>
+__device__ __forceinline__ void fixed_velocity_bgk(A (&f)[Q], const DynEntry<A>& e, const A (&u)[3], A& rho)
+{
+  A s = A(0);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) s += f[i];
+  rho = s + A(1);
+  const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    f[i] = f[i] + e.omega * (feq_i(i, rho, u, usqr) - f[i]);
+  }
+}
+
 template <typename A>
 __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
                                                         const StepArgs<A>& a, A& st_rho, A& st_usqr)
@@ -777,9 +809,7 @@ __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntr
   const A u[3] = {wall_component(a, e, n, 0), wall_component(a, e, n, 1), wall_component(a, e, n, 2)};
   A rho;
   if (e.kind == OLBGFX_LOCAL_VELOCITY) {
-    int dir, orient;
-    flat_normal(a.normal[n], dir, orient);
-    local_velocity(f, dir, orient, u, e.omega, rho);
+    face_local_velocity(f, e, n, a, u, rho);
   } else if (e.kind == OLBGFX_LOCAL_PRESSURE) {
     int dir, orient;
     flat_normal(a.normal[n], dir, orient);
@@ -816,16 +846,25 @@ __device__ __forceinline__ bool apply_velocity_boundary(A (&f)[Q], const DynEntr
     const int N[3] = {normal_component(code, 0), normal_component(code, 1), normal_component(code, 2)};
     inner_velocity(f, N, u, e.omega, rho);
   } else {
-    // BulkDensity + FixedVelocityMomentumGeneric, BGK (collision.h:57-80)
-    A s = A(0);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) s += f[i];
-    rho = s + A(1);
-    const A usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      f[i] = f[i] + e.omega * (feq_i(i, rho, u, usqr) - f[i]);
-    }
+    fixed_velocity_bgk(f, e, u, rho);
+  }
+  st_rho = rho;
+  st_usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  return true;
+}
+
+// the two kinds the dominant launch of a split step updates inline
+// (kWithFaces), with the arithmetic of apply_velocity_boundary
+template <typename A>
+__device__ __forceinline__ bool apply_face_velocity(A (&f)[Q], const DynEntry<A>& e, uint32_t n,
+                                                    const StepArgs<A>& a, A& st_rho, A& st_usqr)
+{
+  const A u[3] = {wall_component(a, e, n, 0), wall_component(a, e, n, 1), wall_component(a, e, n, 2)};
+  A rho;
+  if (e.kind == OLBGFX_LOCAL_VELOCITY) {
+    face_local_velocity(f, e, n, a, u, rho);
+  } else {
+    fixed_velocity_bgk(f, e, u, rho);
   }
   st_rho = rho;
   st_usqr = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -921,7 +960,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
   int kind;
   int id;
   A f[Q];
-  if constexpr (MODE >= 0) {
+  if constexpr (MODE >= kWithFaces) {
+    id = (int)a.mask[n];
+    kind = a.table[id].kind;
+    if (id != a.uniform_id && kind != OLBGFX_LOCAL_VELOCITY && kind != OLBGFX_FIXED_VELOCITY) {
+      return;
+    }
+  } else if constexpr (MODE >= 0) {
     id = a.uniform_id;
     kind = MODE;
     // the dominant launch of a split step: other cells belong to the
@@ -940,7 +985,13 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
     kind = a.table[id].kind;
   }
   if (kind != OLBGFX_NO_DYNAMICS) {
-    if constexpr (MODE >= 0) {
+    if constexpr (MODE >= kWithFaces) {
+      if (id == a.uniform_id) {
+        counted = apply_kind<MODE - kWithFaces>(f, a.table[id], n, a, st_rho, st_usqr);
+      } else {
+        counted = apply_face_velocity(f, a.table[id], n, a, st_rho, st_usqr);
+      }
+    } else if constexpr (MODE >= 0) {
       counted = apply_kind<MODE>(f, a.table[id], n, a, st_rho, st_usqr);
     } else if constexpr (MODE == kMaskedBoundary) {
       const DynEntry<A> e = a.table[id];
@@ -954,13 +1005,14 @@ __device__ __forceinline__ void step_cell(const StepArgs<typename Storage<S>::A>
       counted = apply_any(f, e, n, a, st_rho, st_usqr);
     }
 
-    // Masked kernels store through slots computed again from an opaque copy
-    // of n, so the 19 load offsets need not stay live across the collision:
+    // Masked kernels (and the dominant launch with the faces inline) store
+    // through slots computed again from an opaque copy of n, so the 19 load
+    // offsets need not stay live across the collision:
     // 65 VGPRs against 103, up to 8 waves per SIMD instead of 4, +17 % on the
     // FP16 porous channel.  The uniform kernels keep their offsets (no gain
     // measured there).
     uint32_t m = n;
-    if constexpr (MODE < 0) {
+    if constexpr (MODE < 0 || MODE >= kWithFaces) {
       asm("" : "+v"(m));
     }
 #pragma unroll
@@ -1810,6 +1862,7 @@ struct olbgfx_lattice {
   // split step: the dominant dynamics runs its own kernel over all core
   // cells (skipping the others), the residual cells a masked kernel over a list
   int32_t split_id = -1;             // dominant table slot, -1: no split
+  bool split_faces = false;          // the dominant launch updates the velocity faces inline
   uint32_t* residual_dev = nullptr;  // linear core indices of the other cells
   uint32_t residual_n = 0;
   uint64_t split_version = ~0ull;
@@ -1910,6 +1963,28 @@ void launch_mode(int mode, dim3 grid, dim3 block, size_t lds, hipStream_t stream
     step_kernel<S, kMaskedBoundary, STATS><<<grid, block, lds, stream>>>(a);
     break;
   default:
+    if constexpr (!STATS) {
+      // split steps never carry statistics
+      switch (mode) {
+      case kWithFaces + OLBGFX_BGK:
+        step_kernel<S, kWithFaces + OLBGFX_BGK, false><<<grid, block, lds, stream>>>(a);
+        return;
+      case kWithFaces + OLBGFX_TRT:
+        step_kernel<S, kWithFaces + OLBGFX_TRT, false><<<grid, block, lds, stream>>>(a);
+        return;
+      case kWithFaces + OLBGFX_FORCED_BGK:
+        step_kernel<S, kWithFaces + OLBGFX_FORCED_BGK, false><<<grid, block, lds, stream>>>(a);
+        return;
+      case kWithFaces + OLBGFX_FORCED_TRT:
+        step_kernel<S, kWithFaces + OLBGFX_FORCED_TRT, false><<<grid, block, lds, stream>>>(a);
+        return;
+      case kWithFaces + OLBGFX_SMAGORINSKY_BGK:
+        step_kernel<S, kWithFaces + OLBGFX_SMAGORINSKY_BGK, false><<<grid, block, lds, stream>>>(a);
+        return;
+      default:
+        break;
+      }
+    }
     step_kernel<S, -1, STATS><<<grid, block, lds, stream>>>(a);
     break;
   }
@@ -1965,13 +2040,24 @@ olbgfx_status launch_post(olbgfx_lattice* L, StepArgs<typename Storage<S>::A>& a
 
 // the dominant dynamics and the residual list of a split step, recomputed
 // when dynamics change.  Split when the lattice holds velocity-boundary kinds
-// (whose masked kernel carries their code and runs at 3 waves per SIMD), the
-// most frequent core dynamics has a kernel of its own (BGK, TRT, their
+// (whose masked kernel carries all their code and runs at 4 waves per SIMD),
+// the most frequent core dynamics has a kernel of its own (BGK, TRT, their
 // forced forms, Smagorinsky BGK) and the other core cells are at most a
-// quarter.  Without boundary kinds the masked kernel is nearly as fast as the
-// dominant one, and the residual launch loses more than the split gains:
-// a wall cell at the end of a z row touches 19 cache lines of its own there,
-// which one launch shares with the row's other cells.
+// quarter.  Without boundary kinds one masked launch is faster than any split
+// measured: the bounce-back cavity at 100^3 fell from 33.2 to 24.5 GLUPS with
+// a residual list (its z-face wall cells, one per z row, each touch 19 cache
+// lines of their own), and a launch specialised to the dominant kind with
+// the bounce-back cells inline ran the porous channel 3-7 % and the
+// bounce-back cavity 1-6 % below one masked launch.
+//
+// With 4- and 8-byte storage the dominant launch also updates the
+// LocalVelocity faces and the FixedVelocity outer edges and corners inline,
+// so that their cells share cache lines with their rows, and the residual
+// list holds the other kinds only; with 2-byte storage they stay in the list
+// and the dominant launch is the uniform kernel skipping them.  Measured on
+// the LocalVelocity cavity at 100^3: inline, 56.1 against 59.9 us per step in
+// FP32; listed, 48.4 against 51.9 us in FP16, whose dominant kernel is
+// partly issue-bound and slows with the boundary arithmetic inline.
 olbgfx_status prepare_split(olbgfx_lattice* L)
 {
   if (L->split_version == L->version) {
@@ -2009,14 +2095,22 @@ olbgfx_status prepare_split(olbgfx_lattice* L)
   if (!own_kernel || 4 * (ncore - count[dom]) > ncore) {
     return OLBGFX_OK;
   }
+  const bool faces = L->elem >= 4;
+  bool inline_id[OLBGFX_MAX_DYNAMICS];
+  uint64_t listed = 0;
+  for (int id = 0; id < OLBGFX_MAX_DYNAMICS; ++id) {
+    const int k = L->table[id].kind;
+    inline_id[id] = id == dom || (faces && (k == OLBGFX_LOCAL_VELOCITY || k == OLBGFX_FIXED_VELOCITY));
+    listed += inline_id[id] ? 0 : count[id];
+  }
   std::vector<uint32_t> rest;
-  rest.reserve(ncore - count[dom]);
+  rest.reserve(listed);
   uint32_t lin = 0;
   for (int x = 0; x < D.nx; ++x) {
     for (int y = 0; y < D.ny; ++y) {
       const size_t base = ((size_t)(x + D.padding) * L->py + (y + D.padding)) * L->pz + D.padding;
       for (int z = 0; z < D.nz; ++z, ++lin) {
-        if (L->host_mask[base + z] != dom) {
+        if (!inline_id[L->host_mask[base + z]]) {
           rest.push_back(lin);
         }
       }
@@ -2028,6 +2122,7 @@ olbgfx_status prepare_split(olbgfx_lattice* L)
   }
   L->residual_n = (uint32_t)rest.size();
   L->split_id = dom;
+  L->split_faces = faces;
   return OLBGFX_OK;
 }
 
@@ -2066,7 +2161,8 @@ olbgfx_status launch_steps(olbgfx_lattice* L, uint32_t steps, hipStream_t stream
     a.skip_others = 1;
     rest.uniform_id = -1;
   }
-  const int dominant = split ? L->table[L->split_id].kind : mode;
+  const int dominant =
+      split ? (L->split_faces ? kWithFaces : 0) + L->table[L->split_id].kind : mode;
   for (uint32_t k = 0; k < steps; ++k) {
     for (int i = 0; i < Q; ++i) a.shift[i] = L->shift[i];
     if (L->stats_on) {

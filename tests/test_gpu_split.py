@@ -1,47 +1,74 @@
 """Split steps: a lattice with velocity-boundary cells whose core is mostly
-one dynamics steps in two launches, that dynamics' own kernel (skipping the
-other cells) and the masked kernel over the list of the other cells; with
-the split switched off one masked launch does all.  Both update every core
-cell once with the same per-cell arithmetic (a*b+c fused only within one
-source expression), so they agree bit for bit in FP32, BF16 and FP64
-storage; in FP16 storage the dominant kernel may fuse an FMA with the store's
-narrowing (one rounding) where the masked kernel rounds twice, so the two
-agree to the last bit of FP16 there.  The split step matches the oracle."""
+one bulk dynamics steps in that dynamics' own kernel (with 4- or 8-byte
+storage also updating the LocalVelocity faces and FixedVelocity edges and
+corners inline) plus a masked launch over a list of the other cells; with
+the split switched off, and on lattices without boundary kinds, one masked
+launch does all.  Both update every core cell
+once with the same per-cell arithmetic (a*b+c fused only within one source
+expression), so they agree bit for bit in FP32, BF16 and FP64 storage; in FP16
+storage one kernel may fuse an FMA with the store's narrowing (one rounding)
+where the other rounds twice, so the two agree to the last bit of FP16 there.
+The split step matches the oracle."""
 import numpy as np
 import pytest
 
-from openlb_amd import BGK, Lattice, cases
+from openlb_amd import BGK, TRT, Dynamics, Lattice, cases
 from oracle.oracle import OracleLattice
 from tests.helpers import rel_err_unshifted
 
 pytestmark = pytest.mark.gpu
 
 
-def _case(which, res, dtype):
+def _case(which, dtype):
     if which == "bounceback":
-        return cases.cavity3d(res, u_lid=0.1, dtype=dtype)
-    return cases.cavity3d_local_velocity(res, u_lid=0.1, dtype=dtype)
+        return cases.cavity3d(40, u_lid=0.1, dtype=dtype)
+    if which == "porous":
+        return cases.porous_channel((48, 24, 24), radius=4.0, porosity=0.7, seed=5, force=1e-5, dtype=dtype)
+    return cases.cavity3d_local_velocity(40, u_lid=0.1, dtype=dtype)
 
 
+def _obstacle(lat, c):
+    """a 4x4x4 block of TRT cells inside the cavity: a kind the dominant
+    launch does not update, so the residual list holds it"""
+    p = c.padding
+    mask = c.dynamics_mask().reshape(c.materials.shape).copy()
+    slot = int(mask.max()) + 1
+    lat.set_dynamics(slot, Dynamics(TRT, omega=1.0 / 0.7, magic=0.25))
+    mask[p + 10:p + 14, p + 12:p + 16, p + 18:p + 22] = slot
+    lat.set_dynamics_mask(mask.reshape(-1))
+
+
+CASES = [("local_velocity", False), ("local_velocity", True), ("bounceback", False), ("porous", False)]
+
+
+@pytest.mark.parametrize("which,obstacle", CASES, ids=["lv", "lv_obstacle", "bounceback", "porous"])
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f64"])
-def test_split_step_equals_one_masked_launch(storage):
-    """LocalVelocity cavity, 60 steps: split and one launch bit for bit equal
-    on every population of the lattice (FP16: within four FP16 ulps, 2e-3,
-    relative to the unshifted populations: the last-bit roundings where one
-    kernel fuses a store's narrowing and the other does not, carried for 60
-    steps; measured 4.9e-4)"""
-    print(f"[split] local_velocity {storage}", flush=True)
+def test_split_step_equals_one_masked_launch(storage, which, obstacle):
+    """60 steps, split against one masked launch: bit for bit equal on every
+    population of the lattice (FP16: within four FP16 ulps, 2e-3, relative to
+    the unshifted populations: last-bit roundings carried for 60 steps)"""
+    print(f"[split] {which} {storage} obstacle={obstacle}", flush=True)
     dtype = np.float64 if storage == "f64" else np.float32
-    c = _case("local_velocity", 40, dtype)
+    c = _case(which, dtype)
     a = Lattice(c.extent, c.padding, c.periodic, storage)
     c.apply(a)
     b = Lattice(c.extent, c.padding, c.periodic, storage)
     c.apply(b)
+    if obstacle:
+        _obstacle(a, c)
+        _obstacle(b, c)
     b.set_split(False)
-    kind, rest = a.get_split()
     p = c.padding
-    walls = int(np.sum(c.materials[p:-p, p:-p, p:-p] != 1))
-    assert kind == BGK and 0 < rest <= walls
+    core = c.materials[p:-p, p:-p, p:-p]
+    if which == "local_velocity":
+        # 4- and 8-byte storage: the faces, edges and corners run inline, the
+        # list holds the obstacle only; 2-byte storage: it holds them all
+        walls = int(np.sum(core != 1))
+        listed = (64 if obstacle else 0) + (walls if storage in ("f16", "bf16") else 0)
+        assert a.get_split() == (BGK, listed)
+    else:
+        # no velocity-boundary kinds: one masked launch either way
+        assert a.get_split() == (-1, 0)
     assert b.get_split() == (-1, 0)
     a.collide_and_stream(60)
     b.collide_and_stream(60)
@@ -53,23 +80,17 @@ def test_split_step_equals_one_masked_launch(storage):
     b.close()
 
 
-def test_split_step_only_with_boundary_kinds():
-    """a bounce-back cavity (no velocity-boundary kinds) and a uniform
-    lattice step in one launch"""
-    c = _case("bounceback", 24, np.float32)
-    a = Lattice(c.extent, c.padding, c.periodic, "f32")
-    c.apply(a)
-    assert a.get_split() == (-1, 0)
+def test_uniform_lattice_takes_one_launch():
+    """a uniform lattice runs the uniform kernel alone"""
     t = cases.tgv(16, dtype=np.float32)
     u = Lattice(t.extent, t.padding, t.periodic, "f32")
     t.apply(u)
     assert u.get_split() == (-1, 0)
-    a.close()
     u.close()
 
 
 def test_split_step_matches_oracle():
-    c = _case("local_velocity", 24, np.float64)
+    c = cases.cavity3d_local_velocity(24, u_lid=0.1, dtype=np.float64)
     gpu = Lattice(c.extent, c.padding, c.periodic, "f64")
     c.apply(gpu)
     orc = OracleLattice(c.extent, c.padding, c.periodic, "f64")
