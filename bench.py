@@ -324,14 +324,16 @@ def probe_ceilings(size, storages, orders):
     with the step kernel's indexing, block shape and (tuned) block order,
     after one step of ring shifts; ring_inplace_aligned is the in-place
     pattern with every ring offset zero (each ring's lines aligned with the
-    cell index), the most any alignment of the rings could give"""
+    cell index), the most any alignment of the rings could give;
+    ring_two_buffer reads the rings of one buffer and writes those of a
+    second at offset zero, the traffic of a two-buffer step"""
     from openlb_amd import probe_bandwidth
 
     out = {}
     for st in storages:
         order = orders.get(st, -1)
         out[st] = {pat: round(probe_bandwidth((size, size, size), 3, st, pat, 1, order, 10), 1)
-                   for pat in ("copy", "ring_read", "ring_write", "ring_inplace")}
+                   for pat in ("copy", "ring_read", "ring_write", "ring_inplace", "ring_two_buffer")}
         out[st]["ring_inplace_aligned"] = round(
             probe_bandwidth((size, size, size), 3, st, "ring_inplace", 0, order, 10), 1)
         out[st]["block_order"] = order
@@ -469,7 +471,8 @@ def main():
         probes = probe_ceilings(args.size, [head_storage] + (["f16"] if "f16" in res and head_storage != "f16" else []),
                                 {k: v["launch"]["order"] for k, v in res.items()})
         print("# probes: " + "; ".join(f"{k} " + ", ".join(f"{p} {v[p]:.0f}" for p in
-                                                           ("copy", "ring_read", "ring_write", "ring_inplace"))
+                                                           ("copy", "ring_read", "ring_write", "ring_inplace",
+                                                            "ring_two_buffer"))
                                        for k, v in probes.items()) + " GB/s", file=sys.stderr)
     cases = None if args.no_openlb_cases else openlb_cases()
     if cases:

@@ -34,6 +34,11 @@ extern "C" {
 /* every core cell loads its 19 populations and stores them back scaled, in
  * place: the step kernel's traffic without its arithmetic; 2 x 19 elements */
 #define OLBGFX_PROBE_RING_INPLACE  3
+/* every core cell loads its 19 populations from the rings of one buffer and
+ * stores them, scaled, into the rings of a second buffer at slot n (offset
+ * zero): the traffic of a two-buffer step, whose write offsets are free;
+ * 2 x 19 elements per cell */
+#define OLBGFX_PROBE_RING_TWO_BUFFER 4
 
 /* desc: the lattice shape (extent, padding, storage F32/F16/BF16/F64; the
  * periodic flags are ignored).  shift_steps: the ring offsets are those after

@@ -482,14 +482,15 @@ class CellList:
             pass
 
 
-PROBES = {"copy": 0, "ring_read": 1, "ring_write": 2, "ring_inplace": 3}
+PROBES = {"copy": 0, "ring_read": 1, "ring_write": 2, "ring_inplace": 3, "ring_two_buffer": 4}
 
 
 def probe_bandwidth(extent, padding: int, storage: str, pattern: str, shift_steps: int = 1,
                     block_order: int = -1, reps: int = 10, stream: int | None = None) -> float:
     """GB/s of one access pattern of the step kernel (include/olbgfx_probe.h)
     on a lattice of that shape and storage: a plain 16-byte copy of the rings,
-    the 19 rings read, written, or read-modified-written in place"""
+    the 19 rings read, written, or read-modified-written in place, or read
+    from one buffer and written to a second (ring_two_buffer)"""
     d = _LatticeDesc()
     d.nx, d.ny, d.nz = (int(v) for v in extent)
     d.padding = int(padding)

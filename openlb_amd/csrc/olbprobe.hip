@@ -1,7 +1,8 @@
 // olbprobe.hip -- bandwidth probes of the step kernel's access patterns
-// (declared in include/olbgfx_probe.h): a plain 16-byte copy, and the 19 PS
-// rings read, written, and read-modified-written in place, with the step
-// kernel's cell indexing, block shape and block order.  They give the
+// (declared in include/olbgfx_probe.h): a plain 16-byte copy, the 19 PS
+// rings read, written, and read-modified-written in place, and read from one
+// buffer and written to another, with the step kernel's cell indexing, block
+// shape and block order.  They give the
 // ceilings against which the step kernel's HBM fraction is read: the
 // in-place pattern moves the same bytes as a step, without its arithmetic.
 
@@ -22,6 +23,7 @@ constexpr int kCz[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1
 
 struct RingArgs {
   void* pop;
+  void* pop2;           // second buffer (two-buffer pattern)
   uint64_t stride;      // elements between rings
   uint32_t count;       // ring length = padded cells
   uint32_t shift[Q];
@@ -46,7 +48,7 @@ __device__ __forceinline__ float to_f(E v) { return (float)v; }
 template <typename E>
 __device__ __forceinline__ E from_f(float v) { return (E)v; }
 
-// PATTERN: 1 read, 2 write, 3 in place
+// PATTERN: 1 read, 2 write, 3 in place, 4 read pop and write pop2 at slot n
 template <typename E, int PATTERN>
 __global__ void __launch_bounds__(kThreads) probe_rings(const RingArgs a, float* sink)
 {
@@ -71,6 +73,9 @@ __global__ void __launch_bounds__(kThreads) probe_rings(const RingArgs a, float*
       acc += to_f(*p);
     } else if constexpr (PATTERN == 2) {
       *p = from_f<E>((float)(lin & 7u) * 1e-3f);
+    } else if constexpr (PATTERN == 4) {
+      E* q = (E*)((char*)a.pop2 + ((uint64_t)i * a.stride) * sizeof(E)) + n;
+      *q = from_f<E>(to_f(*p) * 0.999f);
     } else {
       *p = from_f<E>(to_f(*p) * 0.999f);
     }
@@ -162,7 +167,7 @@ extern "C" {
 olbgfx_status olbgfx_probe_bandwidth(const olbgfx_lattice_desc* desc, int32_t pattern, uint32_t shift_steps,
                                      int32_t block_order, uint32_t reps, void* stream, double* gbps)
 {
-  if (!desc || !gbps || reps == 0 || pattern < OLBGFX_PROBE_COPY || pattern > OLBGFX_PROBE_RING_INPLACE ||
+  if (!desc || !gbps || reps == 0 || pattern < OLBGFX_PROBE_COPY || pattern > OLBGFX_PROBE_RING_TWO_BUFFER ||
       block_order < -1 || block_order > 12) {
     return OLBGFX_ERR_INVALID;
   }
@@ -189,7 +194,7 @@ olbgfx_status olbgfx_probe_bandwidth(const olbgfx_lattice_desc* desc, int32_t pa
   if (hipMalloc(&a, bytes) != hipSuccess) {
     return OLBGFX_ERR_NOMEM;
   }
-  if ((pattern == OLBGFX_PROBE_COPY && hipMalloc(&b, bytes) != hipSuccess) ||
+  if (((pattern == OLBGFX_PROBE_COPY || pattern == OLBGFX_PROBE_RING_TWO_BUFFER) && hipMalloc(&b, bytes) != hipSuccess) ||
       hipMalloc(&sink, sizeof(float)) != hipSuccess) {
     (void)hipFree(a);
     (void)hipFree(b);
@@ -202,6 +207,7 @@ olbgfx_status olbgfx_probe_bandwidth(const olbgfx_lattice_desc* desc, int32_t pa
 
   RingArgs r{};
   r.pop = a;
+  r.pop2 = b;
   r.stride = stride;
   r.count = (uint32_t)count;
   const int64_t proj[3] = {(int64_t)(py * pz), (int64_t)pz, 1};
@@ -261,14 +267,17 @@ olbgfx_status olbgfx_probe_bandwidth(const olbgfx_lattice_desc* desc, int32_t pa
       if (pattern == OLBGFX_PROBE_RING_READ) probe_rings<float, 1><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_WRITE) probe_rings<float, 2><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_INPLACE) probe_rings<float, 3><<<grid, block, lds, s>>>(r, sink);
+      if (pattern == OLBGFX_PROBE_RING_TWO_BUFFER) probe_rings<float, 4><<<grid, block, lds, s>>>(r, sink);
     } else if (elem == 2) {
       if (pattern == OLBGFX_PROBE_RING_READ) probe_rings<_Float16, 1><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_WRITE) probe_rings<_Float16, 2><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_INPLACE) probe_rings<_Float16, 3><<<grid, block, lds, s>>>(r, sink);
+      if (pattern == OLBGFX_PROBE_RING_TWO_BUFFER) probe_rings<_Float16, 4><<<grid, block, lds, s>>>(r, sink);
     } else {
       if (pattern == OLBGFX_PROBE_RING_READ) probe_rings<double, 1><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_WRITE) probe_rings<double, 2><<<grid, block, lds, s>>>(r, sink);
       if (pattern == OLBGFX_PROBE_RING_INPLACE) probe_rings<double, 3><<<grid, block, lds, s>>>(r, sink);
+      if (pattern == OLBGFX_PROBE_RING_TWO_BUFFER) probe_rings<double, 4><<<grid, block, lds, s>>>(r, sink);
     }
   };
 
@@ -306,7 +315,7 @@ olbgfx_status olbgfx_probe_bandwidth(const olbgfx_lattice_desc* desc, int32_t pa
   double moved = 0;
   if (pattern == OLBGFX_PROBE_COPY) {
     moved = 2.0 * (double)bytes;
-  } else if (pattern == OLBGFX_PROBE_RING_INPLACE) {
+  } else if (pattern == OLBGFX_PROBE_RING_INPLACE || pattern == OLBGFX_PROBE_RING_TWO_BUFFER) {
     moved = 2.0 * Q * (double)ncore * (double)elem;
   } else {
     moved = (double)Q * (double)ncore * (double)elem;
